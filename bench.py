@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark: ijdDS neighbour list of 1,048,576 fcc atoms at rc = 5 A on one GPU.
+
+A step is one pass of the hot path over the workload with positions already
+resident in HBM: bin -> count -> scan -> fill of i, j, d, D, S (float64, int32
+indices and shifts) into pre-allocated outputs (steady state, as an MD loop
+runs it). The K timed steps are bracketed by torch.cuda.synchronize() and HIP
+events on the stream the kernels run on; each fill launch is bracketed by its
+own events for the kernel-level roofline.
+
+Prints one JSON line. `value` is pairs/s of the whole pipeline; the
+`roofline` object is for the dominant kernel (the fill); `cpu_baseline` is the
+reference's OpenMP CPU path (compiled from its sources into oracle/_ref) on
+the host cores, on the same input.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "matscipy-neighbours_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "pairs/s at 1M atoms rc=5Å (%HBM peak); batched systems/s vs per-system"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fill_fcc1M.json")
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, cell, inv, quantities, reps):
+    """Time the reference CPU core (or the oracle restatement when the
+    compiled reference is absent) on the same input."""
+    import numpy as np
+
+    import oracle
+    import ctypes  # noqa: F401
+    if oracle.ref_available():
+        secs, npairs, threads = oracle.ref_time(quantities, np.zeros(3), cell, inv,
+                                                w["pbc"], w["positions"], w["cutoff"],
+                                                reps + 1)
+        kind = "reference"
+        what = "reference CPU core compiled from its sources (OpenMP, -O3)"
+    else:
+        t = []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            out = oracle.neighbour_list(quantities, np.zeros(3), cell, inv, w["pbc"],
+                                        w["positions"], w["cutoff"])
+            t.append(time.perf_counter() - t0)
+        secs = np.array(t)
+        npairs = len(out["i"])
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        kind = "port"
+        what = "oracle restatement of the reference CPU path (OpenMP, -O3)"
+    med = float(np.median(secs[1:]))
+    return dict(value=npairs / med, unit="pairs/s", cores=int(threads), kind=kind,
+                sample=f"full workload, {reps} timed calls after 1 warm-up, median "
+                       f"{med * 1e3:.1f} ms/call; {what}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--reps", type=int, default=64, help="fcc unit cells per edge")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              "(the neighbour-list engine runs on a single device; the multi-GPU "
+              "sharding is a design in DESIGN.md, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from matscipy_neighbours_amd import _lib, workloads
+
+    lib = _lib.load()
+    _lib.require_gpu()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+
+    _log("generating workload")
+    w = workloads.fcc_bulk(reps=args.reps)
+    N = w["positions"].shape[0]
+    cell = np.ascontiguousarray(w["cell"], dtype=np.float64)
+    inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+    co = np.zeros(3)
+    pbc = np.ascontiguousarray(w["pbc"], dtype=np.int32)
+    pos = torch.from_numpy(w["positions"]).to(dev)
+    plan = _lib.Plan(0)
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    P_ = _lib.ptr
+
+    npairs = ctypes.c_int64(0)
+    _lib.check(lib.mnl_plan_count(plan.handle, sp, P_(co), P_(cell), P_(inv), P_(pbc), N,
+                                  P_(pos), w["cutoff"], None, None, 0, None, -1,
+                                  ctypes.byref(npairs)))
+    P = int(npairs.value)
+    _log(f"N={N} pairs={P}")
+    out_i = torch.empty(P, dtype=torch.int32, device=dev)
+    out_j = torch.empty(P, dtype=torch.int32, device=dev)
+    out_d = torch.empty(P, dtype=torch.float64, device=dev)
+    out_D = torch.empty((P, 3), dtype=torch.float64, device=dev)
+    out_S = torch.empty((P, 3), dtype=torch.int32, device=dev)
+    np_dev = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    def step(ev=None):
+        _lib.check(lib.mnl_plan_count_async(plan.handle, sp, P_(co), P_(cell), P_(inv),
+                                            P_(pbc), N, P_(pos), w["cutoff"], None, None, 0,
+                                            None, -1, P_(np_dev)))
+        if ev is not None:
+            ev[0].record(stream)
+        _lib.check(lib.mnl_plan_fill_async(plan.handle, sp, P, P_(out_i), P_(out_j),
+                                           P_(out_D), P_(out_d), P_(out_S), 0))
+        if ev is not None:
+            ev[1].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _log(f"warmup done ({args.warmup} steps)")
+
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t0.record(stream)
+    for k in range(args.steps):
+        step(evs[k])
+    t1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms_step = t0.elapsed_time(t1) / args.steps
+    fill_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    assert int(np_dev.item()) == P, "pair count changed between steps"
+    _log(f"timed {args.steps} steps: {ms_step:.4f} ms/step (wall {wall * 1e3 / args.steps:.4f}),"
+         f" fill {fill_ms:.4f} ms")
+
+    # Algorithmic bytes. Pipeline: read positions 24 B/atom, write 52 B/pair.
+    # Fill kernel: write 52 B/pair, read each atom's cell-sorted record
+    # (32 B position + 16 B image/index) and its int64 offset once: 56 B/atom.
+    pipe_bytes = 24 * N + 52 * P
+    fill_bytes = 52 * P + 56 * N
+    achieved = fill_bytes / (fill_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        try:
+            with open(TRAFFIC_FILE) as f:
+                traffic = json.load(f).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu:
+        _log("timing the CPU baseline")
+        cpu = cpu_baseline(w, cell, inv, "ijdDS", args.cpu_reps)
+        _log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": METRIC,
+        "value": P / (ms_step * 1e-3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"fcc a=3.615 A, {args.reps}^3 cells ({N} atoms), jitter 0.1 A, "
+                        f"periodic cube {args.reps * 3.615:.2f} A, rc 5.0 A, ijdDS f64, "
+                        "i-sorted, outputs pre-allocated",
+            "atoms": N,
+            "pairs": P,
+        },
+        "hbm_frac_pipeline": pipe_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_pairs<fill>",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "fill_ms": fill_ms,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,229 @@
+/*
+ * mnl — MI355X-native neighbour-list engine, C ABI.
+ *
+ * Drop-in for the device path of matscipy-neighbours'
+ * `neighbour_list("ijdDS", ...)` and its src/libneighbours C++ API. All
+ * compute runs in hand-written gfx950 HIP kernels; there is no CPU fallback:
+ * every entry point returns MNL_ERROR with a message when no GPU is usable.
+ *
+ * Conventions (kept from the reference):
+ *   - error codes: 0 on success, -1 on error; the message is read with
+ *     mnl_last_error() (thread-local here, a GIL-protected global in the
+ *     reference: src/libneighbours/error.hh:17-33);
+ *   - quantities: bit flags i=1, j=2, D=4, d=8, S=16
+ *     (src/libneighbours/types.hh:37-43);
+ *   - cell is row-major with rows = lattice vectors; inv_cell is taken as
+ *     given by the caller (fractional = inv_cell . (r - origin));
+ *   - output pairs are sorted by i, and within one i in the reference's
+ *     traversal order, bit-identical to the reference CPU path;
+ *     D == r[j] - r[i] + S @ cell.
+ *   - pbc is three int32 (0/1) instead of three C++ bools.
+ *
+ * Pointers marked [device] must be device memory on the plan's/result's
+ * device; [host] must be host memory. Inputs are non-owning and need only
+ * live for the duration of the call (plus, for asynchronous plan calls,
+ * until the stream reaches the enqueued work).
+ */
+#ifndef MNL_H
+#define MNL_H
+
+#include <stdint.h>
+
+#include "mnl_dlpack.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MNL_SUCCESS 0
+#define MNL_ERROR (-1)
+
+enum {
+    MNL_Q_FIRST = 1,   /* i */
+    MNL_Q_SECOND = 2,  /* j */
+    MNL_Q_DISTVEC = 4, /* D */
+    MNL_Q_ABSDIST = 8, /* d */
+    MNL_Q_SHIFT = 16   /* S */
+};
+
+/* Cell-grid backend: the reference rule (cell_list.cc:193-196) picks hashed
+   when ncells > max(2^20, 8*nat); the engine additionally picks hashed for
+   grids that are mostly empty. The backend never changes the output. */
+enum { MNL_BACKEND_AUTO = -1, MNL_BACKEND_DENSE = 0, MNL_BACKEND_HASHED = 1 };
+
+/* Message of the last error on this thread ("" if none). */
+const char *mnl_last_error(void);
+
+/* Library version string. */
+const char *mnl_version(void);
+
+/* Number of usable HIP devices (0 if none). Never fails. */
+int mnl_device_count(void);
+
+/* ------------------------------------------------------------------------
+ * One-shot entry points (reference: neighbour_list_gpu.hh).
+ * ---------------------------------------------------------------------- */
+
+typedef struct mnl_result mnl_result;
+
+/*
+ * Build the neighbour list on the GPU and keep the requested quantities in
+ * device memory, owned by the returned result.
+ * Replaces matscipy::neighbour_list_gpu_device
+ * (src/libneighbours/neighbour_list_gpu.hh:96, request fields :49-65).
+ *   positions          [3*nat] f64, [device] if positions_on_device else [host]
+ *   per_atom_cutoff    [nat] radii (pair cutoff = sum) [host] or NULL
+ *   per_type_cutoff_sq [ncutoffs^2] squared cutoffs [host] or NULL
+ *   types              [nat] int32 [host] or NULL
+ *   cutoff             binning / global cutoff (max pair cutoff for arrays)
+ *   device_id          device to run on; -1 = current device
+ */
+int mnl_neighbour_list_device(int quantities, const double origin[3],
+                              const double cell[9], const double inv_cell[9],
+                              const int32_t pbc[3], int32_t nat,
+                              const double *positions, int positions_on_device,
+                              double cutoff, const double *per_atom_cutoff,
+                              const double *per_type_cutoff_sq, int32_t ncutoffs,
+                              const int32_t *types, int device_id,
+                              mnl_result **out);
+
+/*
+ * Per-atom neighbour counts only (coordination), never materialising pairs.
+ * Replaces matscipy::neighbour_count_gpu_device
+ * (src/libneighbours/neighbour_list_gpu.hh:101-102). The result holds
+ * quantity 'c' (int32 [nat]) and npairs.
+ */
+int mnl_neighbour_count_device(const double origin[3], const double cell[9],
+                               const double inv_cell[9], const int32_t pbc[3],
+                               int32_t nat, const double *positions,
+                               int positions_on_device, double cutoff,
+                               const double *per_atom_cutoff,
+                               const double *per_type_cutoff_sq,
+                               int32_t ncutoffs, const int32_t *types,
+                               int device_id, mnl_result **out);
+
+/* Number of pairs in a result. */
+int64_t mnl_result_npairs(const mnl_result *res);
+
+/* Atom count of a result. */
+int32_t mnl_result_nat(const mnl_result *res);
+
+/* Device the result lives on. */
+int mnl_result_device(const mnl_result *res);
+
+/* [device] pointer of one quantity: 'i','j' int32 [P]; 'D' f64 [P,3];
+   'd' f64 [P]; 'S' int32 [P,3]; 'c' int32 [nat] counts. NULL if absent. */
+void *mnl_result_data(const mnl_result *res, char quantity);
+
+/* Copy one quantity to host memory `dst` (sized for the whole array).
+   Together with a host-positions build this is the host-in/host-out call
+   matscipy::neighbour_list_gpu (neighbour_list_gpu.hh:28-34). */
+int mnl_result_copy_to_host(const mnl_result *res, char quantity, void *dst);
+
+/*
+ * Export one quantity as an unversioned DLPack tensor (device kROCM) that
+ * owns its buffer; the result gives the buffer up (the quantity becomes
+ * absent in `res`). The consumer calls tensor->deleter when done.
+ * Replaces the reference's device_capsule export
+ * (language_bindings/python/bind_py_dlpack.cc:66-112).
+ */
+DLManagedTensor *mnl_result_export_dlpack(mnl_result *res, char quantity);
+
+/* Free a result and the device buffers it still owns. NULL is a no-op. */
+void mnl_result_free(mnl_result *res);
+
+/* ------------------------------------------------------------------------
+ * Plan API: reusable device workspace, caller-owned outputs, explicit stream.
+ * This is the zero-copy path used by the Python binding (outputs allocated by
+ * the caller, e.g. from a caching allocator) and by MD loops.
+ * ---------------------------------------------------------------------- */
+
+typedef struct mnl_plan mnl_plan;
+
+/* Create a plan on `device_id` (-1 = current). Allocates only small state;
+   workspaces grow on first use and are reused across calls. */
+int mnl_plan_create(int device_id, mnl_plan **plan);
+void mnl_plan_destroy(mnl_plan *plan);
+
+/* Device of a plan. */
+int mnl_plan_device(const mnl_plan *plan);
+
+/*
+ * Phase 1: bin the atoms, count every atom's neighbours and scan the counts
+ * into pair offsets, on `stream` (a hipStream_t; NULL = default stream).
+ * Waits for the stream and returns the pair count in *npairs.
+ * Same arguments as mnl_neighbour_list_device except that positions,
+ * per_atom_cutoff and types are [device] pointers; per_type_cutoff_sq is
+ * [host]. `backend` is one of MNL_BACKEND_*.
+ * Replaces the count half of count_and_fill
+ * (src/libneighbours/neighbour_list_gpu.cc:264-285) and the binning of
+ * build_device (:309-462).
+ */
+int mnl_plan_count(mnl_plan *plan, void *stream, const double origin[3],
+                   const double cell[9], const double inv_cell[9],
+                   const int32_t pbc[3], int32_t nat, const double *positions,
+                   double cutoff, const double *per_atom_cutoff,
+                   const double *per_type_cutoff_sq, int32_t ncutoffs,
+                   const int32_t *types, int backend, int64_t *npairs);
+
+/*
+ * Phase 2: write the pairs of the last mnl_plan_count into caller [device]
+ * buffers sized for npairs (NULL = quantity not wanted). Asynchronous on
+ * `stream`. Replaces the fill half of count_and_fill
+ * (neighbour_list_gpu.cc:286-305).
+ */
+int mnl_plan_fill(mnl_plan *plan, void *stream, int32_t *first, int32_t *secnd,
+                  double *distvec, double *absdist, int32_t *shift);
+
+/* As mnl_plan_fill, with D and d written as float32 (pairs still decided in
+   float64). No counterpart in the reference: the FP32 output option. */
+int mnl_plan_fill_f32(mnl_plan *plan, void *stream, int32_t *first,
+                      int32_t *secnd, float *distvec, float *absdist,
+                      int32_t *shift);
+
+/*
+ * Asynchronous phase 1 for steady-state loops and graph capture: as
+ * mnl_plan_count but never waits for the stream and never copies from host
+ * memory once the workspaces are large enough (the first build of a given
+ * size, outside capture, sizes them; growing under capture is an error).
+ * The pair count is written to *npairs_device ([device] int64, may be NULL).
+ * Per-type cutoff tables are not capturable.
+ */
+int mnl_plan_count_async(mnl_plan *plan, void *stream, const double origin[3],
+                         const double cell[9], const double inv_cell[9],
+                         const int32_t pbc[3], int32_t nat, const double *positions,
+                         double cutoff, const double *per_atom_cutoff,
+                         const double *per_type_cutoff_sq, int32_t ncutoffs,
+                         const int32_t *types, int backend, int64_t *npairs_device);
+
+/*
+ * Asynchronous phase 2 with a caller-chosen capacity: pairs at positions
+ * >= capacity are not written (compare *npairs_device with the capacity to
+ * detect overflow and retry larger). D and d are float32 when
+ * float32_output is non-zero, float64 otherwise.
+ */
+int mnl_plan_fill_async(mnl_plan *plan, void *stream, int64_t capacity,
+                        int32_t *first, int32_t *secnd, void *distvec,
+                        void *absdist, int32_t *shift, int float32_output);
+
+/* Copy the per-atom neighbour counts of the last count into [device] int32
+   [nat] (coordination; asynchronous on `stream`). */
+int mnl_plan_counts(mnl_plan *plan, void *stream, int32_t *counts);
+
+/* [device] pointer to the plan's int64 [nat+1] pair offsets (CSR row
+   pointers of the i-sorted list; valid until the next count). */
+const int64_t *mnl_plan_offsets(const mnl_plan *plan);
+
+/* Backend chosen by the last count (MNL_BACKEND_DENSE or _HASHED). */
+int mnl_plan_backend(const mnl_plan *plan);
+
+/* Diagnostic, no counterpart in the reference: y = sqrt(x) for n doubles
+   (host buffers) computed by a device kernel, so that tests can pin that the
+   device square root rounds exactly like the host's (d == sqrt(r2)). */
+int mnl_debug_sqrt(const double *x, double *y, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,744 @@
+/*
+ * mnl host core and C ABI (include/mnl.h).
+ *
+ * The host side does what the reference's build_device does on the host
+ * (src/libneighbours/neighbour_list_gpu.cc:309-492): argument checks, the
+ * grid geometry, the backend choice, workspace management and the launch
+ * sequence; all per-atom and per-pair work is in mnl_kernels.hip.
+ *
+ * Differences from the reference's host side, by design:
+ *   - workspaces live in a plan and are reused (no ~15 hipMalloc/hipFree per
+ *     call), outputs may be caller-owned (plan API);
+ *   - one host synchronisation per build (the pair total, to size the
+ *     outputs), none inside the binning;
+ *   - HIP errors are returned as MNL_ERROR with a message instead of
+ *     abort() (device.hh:68-76);
+ *   - the error message is thread-local.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mnl.h"
+#include "mnl_engine.hh"
+
+using namespace mnl;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(const std::string &msg) {
+    g_error = msg;
+    return MNL_ERROR;
+}
+
+#define MNL_HIP(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return fail(std::string("HIP error in ") + #expr + ": " +              \
+                        hipGetErrorString(e_));                                    \
+    } while (0)
+
+/* Switch to a device for the scope; restore the previous one on exit. */
+struct DeviceScope {
+    int prev = -1;
+    bool ok = true;
+    explicit DeviceScope(int dev) {
+        if (dev < 0) return;
+        int cur = 0;
+        if (hipGetDevice(&cur) != hipSuccess) {
+            ok = false;
+            return;
+        }
+        if (cur != dev) {
+            if (hipSetDevice(dev) != hipSuccess) {
+                ok = false;
+                return;
+            }
+            prev = cur;
+        }
+    }
+    ~DeviceScope() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+/* Device buffer that only grows. */
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (ptr) {
+            hipError_t e = hipFree(ptr);
+            if (e != hipSuccess) return e;
+            ptr = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(bytes + bytes / 8, 256);
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(ptr);
+    }
+};
+
+int check_device_available() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return fail("no HIP device available: the mnl engine runs only on the GPU");
+    return MNL_SUCCESS;
+}
+
+bool valid_quantities(int q) { return (q & ~31) == 0; }
+
+}  // namespace
+
+/* ======================================================================== */
+
+struct mnl_plan {
+    int device = 0;
+    /* workspaces */
+    DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, counts,
+        offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
+        htypes;
+    int64_t *host_total = nullptr; /* pinned */
+    /* state of the last count */
+    bool counted = false;
+    int32_t nat = 0;
+    int32_t nsys = 1;
+    int backend = MNL_BACKEND_DENSE;
+    int cut = CUT_GLOBAL;
+    int32_t home_cells = 0;
+    int64_t hmask = 0;
+    int64_t npairs = 0;
+    int32_t ncut = 0;
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+/* The pair-kernel arguments of the plan's last count. */
+PairArgs pair_args(const mnl_plan *p) {
+    PairArgs a{};
+    a.geo = p->geo.as<SysGeom>();
+    a.cell_base = p->cell_base.as<int32_t>();
+    a.nsys = p->nsys;
+    a.home_cells = p->home_cells;
+    a.cell_first = p->cell_first.as<int32_t>();
+    a.cell_count = p->cell_count.as<int32_t>();
+    a.hkey = p->backend == MNL_BACKEND_HASHED ? p->hkey.as<int64_t>() : nullptr;
+    a.hmask = p->hmask;
+    a.spos = p->spos.as<double4>();
+    a.sinfo = p->sinfo.as<int4>();
+    a.stype = p->cut == CUT_PER_TYPE ? p->stype.as<int32_t>() : nullptr;
+    a.pt_sq = p->cut == CUT_PER_TYPE ? p->pt.as<double>() : nullptr;
+    a.ncut = p->ncut;
+    a.atom_base = p->atom_base.as<int32_t>();
+    a.counts = p->counts.as<int32_t>();
+    a.offsets = p->offsets.as<int64_t>();
+    a.capacity = p->npairs;
+    return a;
+}
+
+/*
+ * Enqueue bin -> scan -> scatter -> rank/gather -> count -> scan for one
+ * system. Inputs are device pointers. Leaves the pair total in p->total
+ * (device); does not synchronise.
+ */
+int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
+                  const double cell[9], const double inv_cell[9], const int32_t pbc[3],
+                  int32_t nat, const double *positions, double cutoff,
+                  const double *per_atom, const double *per_type_sq, int32_t ncutoffs,
+                  const int32_t *types, int backend) {
+    p->counted = false;
+    if (nat < 0) return fail("negative atom count");
+    if (!valid_quantities(0)) return fail("bad quantities");
+    if (!(cutoff == cutoff)) return fail("cutoff is NaN");
+
+    SysGeom g{};
+    if (!make_sys_geom(origin, cell, inv_cell, pbc, cutoff, 0, nat, 0, g))
+        return fail("Zero cell volume.");
+    for (int k = 0; k < 3; k++) {
+        if (g.nxyz[k] > 255)
+            return fail("cutoff spans more than 255 cells of the grid in one direction");
+        if (g.nxyz[k] < 0) return fail("negative cutoff");
+    }
+    const int64_t ncells = static_cast<int64_t>(g.n[0]) * g.n[1] * g.n[2];
+    const int64_t budget = std::max<int64_t>(int64_t(1) << 20, int64_t(8) * nat);
+    bool hashed;
+    if (backend == MNL_BACKEND_HASHED)
+        hashed = true;
+    else if (backend == MNL_BACKEND_DENSE)
+        hashed = false;
+    else
+        hashed = ncells > budget; /* cell_list.cc:193-196 */
+    if (!hashed && ncells > INT32_MAX / 2)
+        return fail("cell grid too large for the dense backend");
+
+    p->nat = nat;
+    p->nsys = 1;
+    p->backend = hashed ? MNL_BACKEND_HASHED : MNL_BACKEND_DENSE;
+    p->cut = per_atom ? CUT_PER_ATOM : ((per_type_sq && types) ? CUT_PER_TYPE : CUT_GLOBAL);
+    p->ncut = p->cut == CUT_PER_TYPE ? ncutoffs : 0;
+    p->stream = stream;
+
+    const size_t n = static_cast<size_t>(std::max(nat, 1));
+    int64_t cells_or_slots = ncells;
+    if (hashed) {
+        int64_t cap = 1;
+        while (cap < 2 * static_cast<int64_t>(nat) + 1) cap <<= 1;
+        p->hmask = cap - 1;
+        cells_or_slots = cap;
+    } else {
+        p->hmask = 0;
+    }
+    p->home_cells = static_cast<int32_t>(cells_or_slots);
+
+    /* Workspaces only grow; growing frees the old buffer, so pending work on
+       the stream must finish first. A stream under graph capture cannot wait:
+       the caller runs one uncaptured build of the same size beforehand. */
+    const int64_t nparts =
+        std::max(scan_partial_size(cells_or_slots), scan_partial_size(static_cast<int64_t>(n)));
+    struct Need {
+        DevBuf *b;
+        size_t bytes;
+    } needs[] = {
+        {&p->raw4, n * sizeof(int4)},
+        {&p->ticket, n * sizeof(int32_t)},
+        {&p->tmp, n * sizeof(int32_t)},
+        {&p->spos, n * sizeof(double4)},
+        {&p->sinfo, n * sizeof(int4)},
+        {&p->stype, p->cut == CUT_PER_TYPE ? n * sizeof(int32_t) : 0},
+        {&p->pt, p->cut == CUT_PER_TYPE
+                     ? sizeof(double) * std::max<int64_t>(1, int64_t(ncutoffs) * ncutoffs)
+                     : 0},
+        {&p->counts, n * sizeof(int32_t)},
+        {&p->offsets, (n + 1) * sizeof(int64_t)},
+        {&p->cell_count, static_cast<size_t>(cells_or_slots) * sizeof(int32_t)},
+        {&p->cell_first, static_cast<size_t>(cells_or_slots) * sizeof(int32_t)},
+        {&p->partial, static_cast<size_t>(nparts) * sizeof(int64_t)},
+        {&p->geo, sizeof(SysGeom)},
+        {&p->atom_base, 2 * sizeof(int32_t)},
+        {&p->cell_base, 2 * sizeof(int32_t)},
+        {&p->total, sizeof(int64_t)},
+        {&p->hkey, hashed ? static_cast<size_t>(cells_or_slots) * sizeof(int64_t) : 0},
+    };
+    bool grow = false;
+    for (const Need &nd : needs) grow = grow || nd.bytes > nd.b->cap;
+    hipStreamCaptureStatus cap_status = hipStreamCaptureStatusNone;
+    MNL_HIP(hipStreamIsCapturing(stream, &cap_status));
+    const bool capturing = cap_status != hipStreamCaptureStatusNone;
+    if (grow) {
+        if (capturing)
+            return fail("mnl workspace must grow during graph capture: run one "
+                        "uncaptured build of this size first");
+        MNL_HIP(hipStreamSynchronize(stream));
+        for (const Need &nd : needs) MNL_HIP(nd.b->ensure(nd.bytes));
+    }
+
+    /* parameters are written on the device by a one-thread kernel: no
+       host-to-device copy, nothing read from host memory later */
+    MNL_HIP(launch_set_params(p->geo.as<SysGeom>(), g, p->atom_base.as<int32_t>(),
+                              p->cell_base.as<int32_t>(), nat,
+                              static_cast<int32_t>(hashed ? 0 : ncells), stream));
+    if (p->cut == CUT_PER_TYPE && ncutoffs > 0) {
+        if (capturing) return fail("per-type cutoffs cannot be used under graph capture");
+        const size_t b = sizeof(double) * size_t(ncutoffs) * size_t(ncutoffs);
+        MNL_HIP(hipMemcpyAsync(p->pt.ptr, per_type_sq, b, hipMemcpyHostToDevice, stream));
+        MNL_HIP(hipStreamSynchronize(stream));
+    }
+
+    MNL_HIP(launch_fill_i32(p->cell_count.as<int32_t>(), cells_or_slots, 0, stream));
+    if (hashed) {
+        MNL_HIP(launch_fill_i64(p->hkey.as<int64_t>(), cells_or_slots, -1, stream));
+        MNL_HIP(launch_bin_hashed(positions, nat, p->geo.as<SysGeom>(), p->raw4.as<int4>(),
+                                  p->ticket.as<int32_t>(), p->hkey.as<int64_t>(), p->hmask,
+                                  p->cell_count.as<int32_t>(), stream));
+    } else {
+        MNL_HIP(launch_bin_dense(positions, nat, p->geo.as<SysGeom>(),
+                                 p->atom_base.as<int32_t>(), 1, p->raw4.as<int4>(),
+                                 p->ticket.as<int32_t>(), p->cell_count.as<int32_t>(),
+                                 stream));
+    }
+    MNL_HIP(launch_scan_i32_to_i32(p->cell_count.as<int32_t>(), cells_or_slots,
+                                   p->cell_first.as<int32_t>(), p->partial.as<int64_t>(),
+                                   nullptr, stream));
+    MNL_HIP(launch_scatter(p->raw4.as<int4>(), p->ticket.as<int32_t>(), nat,
+                           p->cell_first.as<int32_t>(), p->tmp.as<int32_t>(), stream));
+    MNL_HIP(launch_rank_gather(positions, p->raw4.as<int4>(), p->tmp.as<int32_t>(), nat,
+                               p->cell_first.as<int32_t>(), p->cell_count.as<int32_t>(),
+                               p->geo.as<SysGeom>(), p->atom_base.as<int32_t>(), 1, per_atom,
+                               types, p->spos.as<double4>(), p->sinfo.as<int4>(),
+                               p->cut == CUT_PER_TYPE ? p->stype.as<int32_t>() : nullptr,
+                               stream));
+    PairArgs a = pair_args(p);
+    MNL_HIP(launch_pairs(0, p->cut, hashed, a, stream));
+    MNL_HIP(launch_scan_i32_to_i64(p->counts.as<int32_t>(), nat, p->offsets.as<int64_t>(),
+                                   p->partial.as<int64_t>(), p->total.as<int64_t>(), stream));
+    return MNL_SUCCESS;
+}
+
+int read_total(mnl_plan *p, hipStream_t stream, int64_t *npairs) {
+    MNL_HIP(hipMemcpyAsync(p->host_total, p->total.ptr, sizeof(int64_t),
+                           hipMemcpyDeviceToHost, stream));
+    MNL_HIP(hipStreamSynchronize(stream));
+    p->npairs = *p->host_total;
+    p->counted = true;
+    if (npairs) *npairs = p->npairs;
+    return MNL_SUCCESS;
+}
+
+int enqueue_fill(mnl_plan *p, hipStream_t stream, int mode, int32_t *first, int32_t *secnd,
+                 void *distvec, void *absdist, int32_t *shift) {
+    if (!p->counted) return fail("mnl_plan_fill called without a successful mnl_plan_count");
+    if (p->npairs < 0) return fail("pair count unknown: use mnl_plan_fill_async with a capacity");
+    if (p->npairs == 0 || p->nat == 0) return MNL_SUCCESS;
+    PairArgs a = pair_args(p);
+    a.out_i = first;
+    a.out_j = secnd;
+    a.out_D = distvec;
+    a.out_d = absdist;
+    a.out_S = shift;
+    MNL_HIP(launch_pairs(mode, p->cut, p->backend == MNL_BACKEND_HASHED, a, stream));
+    return MNL_SUCCESS;
+}
+
+/* One cached plan per device for the one-shot entry points. */
+std::mutex g_plans_mutex;
+std::vector<mnl_plan *> g_plans;
+
+mnl_plan *shared_plan(int device) {
+    if (device < 0) return nullptr;
+    if (static_cast<size_t>(device) >= g_plans.size()) g_plans.resize(device + 1, nullptr);
+    if (!g_plans[device]) {
+        mnl_plan *p = nullptr;
+        if (mnl_plan_create(device, &p) != MNL_SUCCESS) return nullptr;
+        g_plans[device] = p;
+    }
+    return g_plans[device];
+}
+
+}  // namespace
+
+/* ======================================================================== */
+
+struct mnl_result {
+    int device = 0;
+    int32_t nat = 0;
+    int64_t npairs = 0;
+    int quantities = 0;
+    void *buf[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}; /* i j D d S c */
+};
+
+namespace {
+
+int slot_of(char q) {
+    switch (q) {
+        case 'i': return 0;
+        case 'j': return 1;
+        case 'D': return 2;
+        case 'd': return 3;
+        case 'S': return 4;
+        case 'c': return 5;
+        default: return -1;
+    }
+}
+
+size_t bytes_of(const mnl_result *r, int slot) {
+    const size_t P = static_cast<size_t>(r->npairs);
+    switch (slot) {
+        case 0: case 1: return P * 4;
+        case 2: return P * 24;
+        case 3: return P * 8;
+        case 4: return P * 12;
+        case 5: return static_cast<size_t>(r->nat) * 4;
+        default: return 0;
+    }
+}
+
+void free_result_buffers(mnl_result *r) {
+    DeviceScope ds(r->device);
+    for (void *&b : r->buf) {
+        if (b) (void)hipFree(b);
+        b = nullptr;
+    }
+}
+
+int one_shot(int quantities, bool want_pairs, const double origin[3], const double cell[9],
+             const double inv_cell[9], const int32_t pbc[3], int32_t nat,
+             const double *positions, int positions_on_device, double cutoff,
+             const double *per_atom, const double *per_type_sq, int32_t ncutoffs,
+             const int32_t *types, int device_id, mnl_result **out) {
+    if (!out) return fail("out is NULL");
+    *out = nullptr;
+    if (!origin || !cell || !inv_cell || !pbc) return fail("geometry pointer is NULL");
+    if (!valid_quantities(quantities)) return fail("Unsupported quantity specified.");
+    if (nat > 0 && !positions) return fail("positions is NULL");
+    if (per_type_sq && ncutoffs < 0) return fail("negative ncutoffs");
+    if (check_device_available() != MNL_SUCCESS) return MNL_ERROR;
+    int dev = device_id;
+    if (dev < 0 && hipGetDevice(&dev) != hipSuccess) return fail("hipGetDevice failed");
+    DeviceScope ds(dev);
+    if (!ds.ok) return fail("cannot switch to the requested device");
+
+    std::lock_guard<std::mutex> lock(g_plans_mutex);
+    mnl_plan *p = shared_plan(dev);
+    if (!p) return MNL_ERROR;
+    hipStream_t stream = nullptr;
+    const size_t n = static_cast<size_t>(std::max(nat, 1));
+
+    const double *dpos = positions;
+    if (!positions_on_device && nat > 0) {
+        MNL_HIP(p->hpos.ensure(n * 3 * sizeof(double)));
+        MNL_HIP(hipMemcpy(p->hpos.ptr, positions, size_t(nat) * 3 * sizeof(double),
+                          hipMemcpyHostToDevice));
+        dpos = p->hpos.as<double>();
+    }
+    const double *dper = nullptr;
+    if (per_atom && nat > 0) {
+        MNL_HIP(p->hper_atom.ensure(n * sizeof(double)));
+        MNL_HIP(hipMemcpy(p->hper_atom.ptr, per_atom, size_t(nat) * sizeof(double),
+                          hipMemcpyHostToDevice));
+        dper = p->hper_atom.as<double>();
+    }
+    const int32_t *dtypes = nullptr;
+    if (types && nat > 0) {
+        MNL_HIP(p->htypes.ensure(n * sizeof(int32_t)));
+        MNL_HIP(hipMemcpy(p->htypes.ptr, types, size_t(nat) * sizeof(int32_t),
+                          hipMemcpyHostToDevice));
+        dtypes = p->htypes.as<int32_t>();
+    }
+
+    auto *res = new mnl_result();
+    res->device = dev;
+    res->nat = nat;
+    res->quantities = want_pairs ? quantities : 0;
+
+    int64_t np = 0;
+    if (enqueue_count(p, stream, origin, cell, inv_cell, pbc, nat, dpos, cutoff, dper,
+                      per_type_sq, ncutoffs, dtypes, MNL_BACKEND_AUTO) != MNL_SUCCESS ||
+        read_total(p, stream, &np) != MNL_SUCCESS) {
+        delete res;
+        return MNL_ERROR;
+    }
+    res->npairs = np;
+    auto alloc = [&](int slot) -> int {
+        const size_t b = bytes_of(res, slot);
+        if (b == 0) return MNL_SUCCESS;
+        hipError_t e = hipMalloc(&res->buf[slot], b);
+        if (e != hipSuccess) {
+            res->buf[slot] = nullptr;
+            return fail(std::string("hipMalloc of an output failed: ") + hipGetErrorString(e));
+        }
+        return MNL_SUCCESS;
+    };
+    int st = MNL_SUCCESS;
+    if (st == MNL_SUCCESS) st = alloc(5);
+    if (want_pairs) {
+        for (int s = 0; s < 5 && st == MNL_SUCCESS; s++)
+            if (quantities & (1 << s)) st = alloc(s);
+    }
+    if (st == MNL_SUCCESS && nat > 0)
+        st = mnl_plan_counts(p, stream, static_cast<int32_t *>(res->buf[5]));
+    if (st == MNL_SUCCESS && want_pairs)
+        st = enqueue_fill(p, stream, 1, static_cast<int32_t *>(res->buf[0]),
+                          static_cast<int32_t *>(res->buf[1]), res->buf[2], res->buf[3],
+                          static_cast<int32_t *>(res->buf[4]));
+    if (st == MNL_SUCCESS) {
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) st = fail(std::string("HIP error: ") + hipGetErrorString(e));
+    }
+    if (st != MNL_SUCCESS) {
+        free_result_buffers(res);
+        delete res;
+        return MNL_ERROR;
+    }
+    *out = res;
+    return MNL_SUCCESS;
+}
+
+struct DLCtx {
+    void *ptr;
+    int device;
+    int64_t shape[2];
+};
+
+void dl_deleter(DLManagedTensor *t) {
+    if (!t) return;
+    auto *c = static_cast<DLCtx *>(t->manager_ctx);
+    if (c) {
+        DeviceScope ds(c->device);
+        if (c->ptr) (void)hipFree(c->ptr);
+        delete c;
+    }
+    delete t;
+}
+
+}  // namespace
+
+/* ======================================================================== */
+
+extern "C" {
+
+const char *mnl_last_error(void) { return g_error.c_str(); }
+
+const char *mnl_version(void) { return "mnl 0.1.0 (gfx950)"; }
+
+int mnl_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int mnl_plan_create(int device_id, mnl_plan **plan) {
+    if (!plan) return fail("plan is NULL");
+    *plan = nullptr;
+    if (check_device_available() != MNL_SUCCESS) return MNL_ERROR;
+    int dev = device_id;
+    if (dev < 0 && hipGetDevice(&dev) != hipSuccess) return fail("hipGetDevice failed");
+    DeviceScope ds(dev);
+    if (!ds.ok) return fail("cannot switch to the requested device");
+    auto *p = new mnl_plan();
+    p->device = dev;
+    void *h = nullptr;
+    hipError_t e = hipHostMalloc(&h, sizeof(int64_t), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        delete p;
+        return fail(std::string("hipHostMalloc failed: ") + hipGetErrorString(e));
+    }
+    p->host_total = static_cast<int64_t *>(h);
+    *plan = p;
+    return MNL_SUCCESS;
+}
+
+void mnl_plan_destroy(mnl_plan *p) {
+    if (!p) return;
+    DeviceScope ds(p->device);
+    DevBuf *bufs[] = {&p->raw4,      &p->ticket,    &p->tmp,    &p->spos,     &p->sinfo,
+                      &p->stype,     &p->cell_count, &p->cell_first, &p->counts, &p->offsets,
+                      &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
+                      &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes};
+    for (DevBuf *b : bufs) b->release();
+    if (p->host_total) (void)hipHostFree(p->host_total);
+    delete p;
+}
+
+int mnl_plan_device(const mnl_plan *p) { return p ? p->device : -1; }
+
+int mnl_plan_count(mnl_plan *p, void *stream, const double origin[3], const double cell[9],
+                   const double inv_cell[9], const int32_t pbc[3], int32_t nat,
+                   const double *positions, double cutoff, const double *per_atom_cutoff,
+                   const double *per_type_cutoff_sq, int32_t ncutoffs, const int32_t *types,
+                   int backend, int64_t *npairs) {
+    if (!p) return fail("plan is NULL");
+    if (!origin || !cell || !inv_cell || !pbc) return fail("geometry pointer is NULL");
+    if (nat > 0 && !positions) return fail("positions is NULL");
+    DeviceScope ds(p->device);
+    if (!ds.ok) return fail("cannot switch to the plan's device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (enqueue_count(p, s, origin, cell, inv_cell, pbc, nat, positions, cutoff,
+                      per_atom_cutoff, per_type_cutoff_sq, ncutoffs, types,
+                      backend) != MNL_SUCCESS)
+        return MNL_ERROR;
+    return read_total(p, s, npairs);
+}
+
+int mnl_plan_count_async(mnl_plan *p, void *stream, const double origin[3],
+                         const double cell[9], const double inv_cell[9], const int32_t pbc[3],
+                         int32_t nat, const double *positions, double cutoff,
+                         const double *per_atom_cutoff, const double *per_type_cutoff_sq,
+                         int32_t ncutoffs, const int32_t *types, int backend,
+                         int64_t *npairs_device) {
+    if (!p) return fail("plan is NULL");
+    if (!origin || !cell || !inv_cell || !pbc) return fail("geometry pointer is NULL");
+    if (nat > 0 && !positions) return fail("positions is NULL");
+    DeviceScope ds(p->device);
+    if (!ds.ok) return fail("cannot switch to the plan's device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (enqueue_count(p, s, origin, cell, inv_cell, pbc, nat, positions, cutoff,
+                      per_atom_cutoff, per_type_cutoff_sq, ncutoffs, types,
+                      backend) != MNL_SUCCESS)
+        return MNL_ERROR;
+    if (npairs_device)
+        MNL_HIP(hipMemcpyAsync(npairs_device, p->total.ptr, sizeof(int64_t),
+                               hipMemcpyDeviceToDevice, s));
+    p->counted = true;
+    p->npairs = -1; /* unknown on the host until read */
+    return MNL_SUCCESS;
+}
+
+int mnl_plan_fill_async(mnl_plan *p, void *stream, int64_t capacity, int32_t *first,
+                        int32_t *secnd, void *distvec, void *absdist, int32_t *shift,
+                        int float32_output) {
+    if (!p) return fail("plan is NULL");
+    if (!p->counted) return fail("mnl_plan_fill_async called without a count");
+    if (capacity < 0) return fail("negative capacity");
+    DeviceScope ds(p->device);
+    const int64_t saved = p->npairs;
+    p->npairs = capacity;
+    int st = enqueue_fill(p, static_cast<hipStream_t>(stream), float32_output ? 2 : 1, first,
+                          secnd, distvec, absdist, shift);
+    p->npairs = saved;
+    return st;
+}
+
+int mnl_plan_fill(mnl_plan *p, void *stream, int32_t *first, int32_t *secnd, double *distvec,
+                  double *absdist, int32_t *shift) {
+    if (!p) return fail("plan is NULL");
+    DeviceScope ds(p->device);
+    return enqueue_fill(p, static_cast<hipStream_t>(stream), 1, first, secnd, distvec, absdist,
+                        shift);
+}
+
+int mnl_plan_fill_f32(mnl_plan *p, void *stream, int32_t *first, int32_t *secnd,
+                      float *distvec, float *absdist, int32_t *shift) {
+    if (!p) return fail("plan is NULL");
+    DeviceScope ds(p->device);
+    return enqueue_fill(p, static_cast<hipStream_t>(stream), 2, first, secnd, distvec, absdist,
+                        shift);
+}
+
+int mnl_plan_counts(mnl_plan *p, void *stream, int32_t *counts) {
+    if (!p) return fail("plan is NULL");
+    if (!p->counted) return fail("mnl_plan_counts called without a successful count");
+    if (p->nat == 0) return MNL_SUCCESS;
+    if (!counts) return fail("counts is NULL");
+    DeviceScope ds(p->device);
+    MNL_HIP(hipMemcpyAsync(counts, p->counts.ptr, sizeof(int32_t) * size_t(p->nat),
+                           hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)));
+    return MNL_SUCCESS;
+}
+
+const int64_t *mnl_plan_offsets(const mnl_plan *p) {
+    return (p && p->counted) ? p->offsets.as<int64_t>() : nullptr;
+}
+
+int mnl_plan_backend(const mnl_plan *p) { return p ? p->backend : -1; }
+
+int mnl_neighbour_list_device(int quantities, const double origin[3], const double cell[9],
+                              const double inv_cell[9], const int32_t pbc[3], int32_t nat,
+                              const double *positions, int positions_on_device,
+                              double cutoff, const double *per_atom_cutoff,
+                              const double *per_type_cutoff_sq, int32_t ncutoffs,
+                              const int32_t *types, int device_id, mnl_result **out) {
+    return one_shot(quantities, true, origin, cell, inv_cell, pbc, nat, positions,
+                    positions_on_device, cutoff, per_atom_cutoff, per_type_cutoff_sq,
+                    ncutoffs, types, device_id, out);
+}
+
+int mnl_neighbour_count_device(const double origin[3], const double cell[9],
+                               const double inv_cell[9], const int32_t pbc[3], int32_t nat,
+                               const double *positions, int positions_on_device,
+                               double cutoff, const double *per_atom_cutoff,
+                               const double *per_type_cutoff_sq, int32_t ncutoffs,
+                               const int32_t *types, int device_id, mnl_result **out) {
+    return one_shot(0, false, origin, cell, inv_cell, pbc, nat, positions,
+                    positions_on_device, cutoff, per_atom_cutoff, per_type_cutoff_sq,
+                    ncutoffs, types, device_id, out);
+}
+
+int64_t mnl_result_npairs(const mnl_result *r) { return r ? r->npairs : 0; }
+
+int32_t mnl_result_nat(const mnl_result *r) { return r ? r->nat : 0; }
+
+int mnl_result_device(const mnl_result *r) { return r ? r->device : -1; }
+
+void *mnl_result_data(const mnl_result *r, char q) {
+    const int s = slot_of(q);
+    if (!r || s < 0) return nullptr;
+    return r->buf[s];
+}
+
+int mnl_result_copy_to_host(const mnl_result *r, char q, void *dst) {
+    if (!r) return fail("result is NULL");
+    const int s = slot_of(q);
+    if (s < 0) return fail("Unsupported quantity specified.");
+    const size_t b = bytes_of(r, s);
+    if (b == 0) return MNL_SUCCESS;
+    if (!r->buf[s]) return fail(std::string("quantity '") + q + "' is not held by the result");
+    if (!dst) return fail("dst is NULL");
+    DeviceScope ds(r->device);
+    MNL_HIP(hipMemcpy(dst, r->buf[s], b, hipMemcpyDeviceToHost));
+    return MNL_SUCCESS;
+}
+
+DLManagedTensor *mnl_result_export_dlpack(mnl_result *r, char q) {
+    const int s = slot_of(q);
+    if (!r || s < 0) {
+        fail("Unsupported quantity specified.");
+        return nullptr;
+    }
+    const bool present = r->buf[s] != nullptr || bytes_of(r, s) == 0;
+    const bool requested = s == 5 || (r->quantities & (1 << s));
+    if (!present || !requested) {
+        fail(std::string("quantity '") + q + "' is not held by the result");
+        return nullptr;
+    }
+    auto *ctx = new DLCtx();
+    ctx->ptr = r->buf[s];
+    ctx->device = r->device;
+    r->buf[s] = nullptr;
+    if (s < 5) r->quantities &= ~(1 << s);
+    auto *t = new DLManagedTensor();
+    std::memset(t, 0, sizeof *t);
+    const bool vec3 = (s == 2 || s == 4);
+    ctx->shape[0] = s == 5 ? r->nat : r->npairs;
+    ctx->shape[1] = 3;
+    t->dl_tensor.data = ctx->ptr;
+    t->dl_tensor.device.device_type = kDLROCM;
+    t->dl_tensor.device.device_id = r->device;
+    t->dl_tensor.ndim = vec3 ? 2 : 1;
+    t->dl_tensor.dtype.code = (s == 2 || s == 3) ? kDLFloat : kDLInt;
+    t->dl_tensor.dtype.bits = (s == 2 || s == 3) ? 64 : 32;
+    t->dl_tensor.dtype.lanes = 1;
+    t->dl_tensor.shape = ctx->shape;
+    t->dl_tensor.strides = nullptr;
+    t->dl_tensor.byte_offset = 0;
+    t->manager_ctx = ctx;
+    t->deleter = dl_deleter;
+    return t;
+}
+
+void mnl_result_free(mnl_result *r) {
+    if (!r) return;
+    free_result_buffers(r);
+    delete r;
+}
+
+int mnl_debug_sqrt(const double *x_host, double *y_host, int64_t n) {
+    if (n <= 0) return MNL_SUCCESS;
+    if (check_device_available() != MNL_SUCCESS) return MNL_ERROR;
+    void *dx = nullptr, *dy = nullptr;
+    MNL_HIP(hipMalloc(&dx, size_t(n) * 8));
+    MNL_HIP(hipMalloc(&dy, size_t(n) * 8));
+    MNL_HIP(hipMemcpy(dx, x_host, size_t(n) * 8, hipMemcpyHostToDevice));
+    MNL_HIP(launch_sqrt_probe(static_cast<double *>(dx), static_cast<double *>(dy), n, nullptr));
+    MNL_HIP(hipMemcpy(y_host, dy, size_t(n) * 8, hipMemcpyDeviceToHost));
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    return MNL_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+/*
+ * mnl engine internals shared by the host core and the HIP kernels.
+ *
+ * Data layout in HBM (one build, N atoms, C cells):
+ *   raw4   int4   [N]  raw (unwrapped) cell coords + global cell id, atom order
+ *   ticket int32  [N]  arrival slot inside the cell (from the histogram atomic)
+ *   tmp    int32  [N]  atoms scattered into cell slices, arbitrary order
+ *   spos   double4[N]  cell-sorted: bin-relative position xyz + per-atom radius
+ *   sinfo  int4   [N]  cell-sorted: image counts xyz + atom index (system-local)
+ *   stype  int32  [N]  cell-sorted atom type (per-type cutoffs only)
+ *   cell_count / cell_first int32 [C] dense CSR (or hash slots when hashed)
+ *   counts int32  [N]  neighbours per atom, original order
+ *   offsets int64 [N+1] exclusive scan of counts (CSR row pointers)
+ * A "system" is one periodic cell with its atoms; the plain path is a batch
+ * of one system, the batched path concatenates systems (atoms, cells and
+ * pairs) with per-system base offsets.
+ */
+#ifndef MNL_ENGINE_HH
+#define MNL_ENGINE_HH
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mnl {
+
+/* Per-system grid description, device resident, read with scalar loads. */
+struct SysGeom {
+    double inv[9];      /* inv_cell as given: fractional = inv . (r - origin) */
+    double origin[3];
+    double bin[9];      /* bin[3*k + m] = cell[3*k + m] / n[k] (cell edge vectors) */
+    double cutoff_sq;
+    int32_t n[3];       /* cells per lattice direction */
+    int32_t nxyz[3];    /* neighbour cells scanned each way */
+    int32_t pbc[3];
+    int32_t atom_base;  /* first atom of the system in the concatenated arrays */
+    int32_t natoms;
+    int32_t cell_base;  /* first dense cell of the system */
+    int32_t ncells;
+    int32_t nseg;       /* (2nx+1)(2ny+1)(2nz+1) neighbour-cell offsets */
+    int64_t pad;
+};
+
+/* Host geometry (src/libneighbours/cell_list.cc:153-182 and the search range
+   of neighbour_list.cc:176-178), with the reference's operation order.
+   Returns false for a zero-volume cell. */
+bool grid_geometry(const double cell[9], double cutoff, int32_t n[3],
+                   int32_t nxyz[3], double bin[9], double len[3]);
+
+/* Fill a SysGeom for one system. Returns false for a zero-volume cell. */
+bool make_sys_geom(const double origin[3], const double cell[9],
+                   const double inv_cell[9], const int32_t pbc[3], double cutoff,
+                   int32_t atom_base, int32_t natoms, int32_t cell_base,
+                   SysGeom &g);
+
+enum CutMode { CUT_GLOBAL = 0, CUT_PER_ATOM = 1, CUT_PER_TYPE = 2 };
+
+/* Arguments of the pair kernels (count and fill), passed by value. */
+struct PairArgs {
+    const SysGeom *geo;
+    const int32_t *cell_base;  /* [nsys+1] */
+    int32_t nsys;
+    int32_t home_cells;        /* dense: total cells; hashed: slot capacity */
+    const int32_t *cell_first;
+    const int32_t *cell_count;
+    const int64_t *hkey;
+    int64_t hmask;
+    const double4 *spos;
+    const int4 *sinfo;
+    const int32_t *stype;
+    const double *pt_sq;
+    int32_t ncut;
+    int32_t xcd_blocks;        /* grid size, for the XCD-aware block remap */
+    const int32_t *atom_base;  /* [nsys+1] */
+    int32_t *counts;           /* count: [N] by global atom index */
+    const int64_t *offsets;    /* fill: [N+1] */
+    int64_t capacity;          /* fill: pairs beyond this are not written */
+    int32_t *out_i;
+    int32_t *out_j;
+    void *out_D;
+    void *out_d;
+    int32_t *out_S;
+};
+
+/* Kernel launchers (mnl_kernels.hip). All asynchronous on `stream`. */
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t stream);
+hipError_t launch_fill_i64(int64_t *p, int64_t n, int64_t v, hipStream_t stream);
+hipError_t launch_bin_dense(const double *r, int32_t nat, const SysGeom *geo,
+                            const int32_t *atom_base, int32_t nsys, int4 *raw4,
+                            int32_t *ticket, int32_t *cell_count,
+                            hipStream_t stream);
+hipError_t launch_bin_hashed(const double *r, int32_t nat, const SysGeom *geo,
+                             int4 *raw4, int32_t *ticket, int64_t *hkey,
+                             int64_t hmask, int32_t *hcount, hipStream_t stream);
+hipError_t launch_scatter(const int4 *raw4, const int32_t *ticket, int32_t nat,
+                          const int32_t *cell_first, int32_t *tmp,
+                          hipStream_t stream);
+hipError_t launch_rank_gather(const double *r, const int4 *raw4, const int32_t *tmp,
+                              int32_t nat, const int32_t *cell_first,
+                              const int32_t *cell_count, const SysGeom *geo,
+                              const int32_t *atom_base, int32_t nsys,
+                              const double *per_atom, const int32_t *types,
+                              double4 *spos, int4 *sinfo, int32_t *stype,
+                              hipStream_t stream);
+/* Exclusive scan of n int32 values; out gets n (+1 with the total) entries.
+   partial needs scan_partial_size(n) int64 entries. total (device, may be
+   NULL) receives the sum. */
+int64_t scan_partial_size(int64_t n);
+hipError_t launch_scan_i32_to_i32(const int32_t *in, int64_t n, int32_t *out,
+                                  int64_t *partial, int64_t *total,
+                                  hipStream_t stream);
+hipError_t launch_scan_i32_to_i64(const int32_t *in, int64_t n, int64_t *out,
+                                  int64_t *partial, int64_t *total,
+                                  hipStream_t stream);
+/* Pair kernels. mode 0 = count, 1 = fill f64 outputs, 2 = fill f32 outputs. */
+hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
+                        hipStream_t stream);
+hipError_t launch_set_params(SysGeom *geo, const SysGeom &g, int32_t *atom_base,
+                             int32_t *cell_base, int32_t nat, int32_t ncells,
+                             hipStream_t stream);
+hipError_t launch_sqrt_probe(const double *x, double *y, int64_t n,
+                             hipStream_t stream);
+
+}  // namespace mnl
+
+#endif

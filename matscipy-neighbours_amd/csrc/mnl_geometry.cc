@@ -1,0 +1,80 @@
+/*
+ * Host geometry of the cell grid. Built with -ffp-contract=off: the cell
+ * resolution, cell edge vectors and search range must come out bit-identical
+ * to the reference's host arithmetic, since they feed every position the
+ * kernels compute.
+ */
+#include <cmath>
+
+#include "mnl_engine.hh"
+
+#pragma clang fp contract(off)
+
+namespace mnl {
+
+namespace {
+
+/* c = a x b (src/libneighbours/tools.cc:17-21). */
+void cross3(const double *a, const double *b, double *c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* |a| (src/libneighbours/tools.cc:23-25). */
+double norm3(const double *a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+
+}  // namespace
+
+/* src/libneighbours/cell_list.cc:153-182: volume from the triple product,
+   widths = volume / |face normal|, n = max(floor(width / cutoff), 1),
+   edge vectors = lattice vector / n; plus neighbour_list.cc:176-178:
+   nx = ceil(cutoff * n / width). */
+bool grid_geometry(const double cell[9], double cutoff, int32_t n[3],
+                   int32_t nxyz[3], double bin[9], double len[3]) {
+    const double *c1 = &cell[0], *c2 = &cell[3], *c3 = &cell[6];
+    double nrm[3][3];
+    cross3(c2, c3, nrm[0]);
+    cross3(c3, c1, nrm[1]);
+    cross3(c1, c2, nrm[2]);
+    const double volume =
+        std::fabs(c3[0] * nrm[2][0] + c3[1] * nrm[2][1] + c3[2] * nrm[2][2]);
+    if (!(volume >= 1e-12)) return false;
+    for (int k = 0; k < 3; k++) {
+        len[k] = volume / norm3(nrm[k]);
+        const int32_t nk = static_cast<int32_t>(std::floor(len[k] / cutoff));
+        n[k] = nk > 1 ? nk : 1;
+    }
+    for (int m = 0; m < 3; m++) {
+        bin[0 + m] = c1[m] / n[0];
+        bin[3 + m] = c2[m] / n[1];
+        bin[6 + m] = c3[m] / n[2];
+    }
+    for (int k = 0; k < 3; k++)
+        nxyz[k] = static_cast<int32_t>(std::ceil(cutoff * n[k] / len[k]));
+    return true;
+}
+
+bool make_sys_geom(const double origin[3], const double cell[9],
+                   const double inv_cell[9], const int32_t pbc[3], double cutoff,
+                   int32_t atom_base, int32_t natoms, int32_t cell_base,
+                   SysGeom &g) {
+    double len[3];
+    if (!grid_geometry(cell, cutoff, g.n, g.nxyz, g.bin, len)) return false;
+    for (int k = 0; k < 9; k++) g.inv[k] = inv_cell[k];
+    for (int k = 0; k < 3; k++) {
+        g.origin[k] = origin[k];
+        g.pbc[k] = pbc[k] ? 1 : 0;
+    }
+    g.cutoff_sq = cutoff * cutoff;
+    g.atom_base = atom_base;
+    g.natoms = natoms;
+    g.cell_base = cell_base;
+    const int64_t nc = static_cast<int64_t>(g.n[0]) * g.n[1] * g.n[2];
+    g.ncells = nc > INT32_MAX ? INT32_MAX : static_cast<int32_t>(nc);
+    g.nseg = (2 * g.nxyz[0] + 1) * (2 * g.nxyz[1] + 1) * (2 * g.nxyz[2] + 1);
+    g.pad = 0;
+    return true;
+}
+
+}  // namespace mnl

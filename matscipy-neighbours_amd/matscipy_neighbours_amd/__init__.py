@@ -1,0 +1,27 @@
+"""matscipy_neighbours_amd - MI355X-native neighbour lists.
+
+A drop-in for the public API of ``matscipy_neighbours`` (itself compatible
+with ``matscipy.neighbours``) whose every build runs in hand-written gfx950
+HIP kernels (libmnl.so, C ABI in include/mnl.h). Pairs, their order and the
+bits of D and d are identical to the reference CPU path.
+"""
+
+from .neighbours import (
+    DLPackTensor,
+    coordination,
+    first_neighbours,
+    get_jump_indicies,
+    mic,
+    neighbour_list,
+    triplet_list,
+)
+
+__all__ = [
+    "neighbour_list",
+    "first_neighbours",
+    "get_jump_indicies",
+    "triplet_list",
+    "mic",
+    "coordination",
+    "DLPackTensor",
+]

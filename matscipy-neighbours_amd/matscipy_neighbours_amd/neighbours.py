@@ -1,0 +1,451 @@
+"""Public neighbour-list API, a drop-in for ``matscipy_neighbours``.
+
+Mirrors language_bindings/python/matscipy_neighbours/neighbours.py of the
+reference: same functions, argument meaning, return layout and error types.
+The computation always runs in the mnl HIP engine (libmnl.so) on the GPU;
+PyTorch only provides device memory and the current stream. There is no CPU
+code path: without a HIP device every call raises RuntimeError.
+
+Differences from the reference, all deliberate:
+
+* host (numpy) inputs are computed on the GPU too and come back as numpy
+  arrays, like the reference's host fast path; ``device="cpu"`` asks for
+  host-resident outputs, it does not select a CPU implementation;
+* device outputs are torch tensors by default (the reference returns cupy,
+  which is absent from ROCm PyTorch images);
+* ``out_dtype=numpy.float32`` (or torch.float32) writes D and d in float32;
+  the pair set is still decided in float64.
+"""
+
+import threading
+
+import numpy as np
+
+from . import _lib
+
+__all__ = [
+    "neighbour_list",
+    "coordination",
+    "first_neighbours",
+    "get_jump_indicies",
+    "triplet_list",
+    "mic",
+    "DLPackTensor",
+]
+
+_DLPACK_CPU = 1
+_DLPACK_CUDA = 2
+_DLPACK_ROCM = 10
+
+# Chemical symbols by atomic number, for element-pair cutoff dicts
+# (the reference uses ase.data.atomic_numbers when ASE is installed).
+_SYMBOLS = (
+    "X H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca Sc Ti V Cr Mn Fe Co "
+    "Ni Cu Zn Ga Ge As Se Br Kr Rb Sr Y Zr Nb Mo Tc Ru Rh Pd Ag Cd In Sn Sb Te "
+    "I Xe Cs Ba La Ce Pr Nd Pm Sm Eu Gd Tb Dy Ho Er Tm Yb Lu Hf Ta W Re Os Ir "
+    "Pt Au Hg Tl Pb Bi Po At Rn Fr Ra Ac Th Pa U Np Pu Am Cm Bk Cf Es Fm Md No "
+    "Lr Rf Db Sg Bh Hs Mt Ds Rg Cn Nh Fl Mc Lv Ts Og").split()
+try:  # pragma: no cover - ase is optional
+    from ase.data import atomic_numbers as _atomic_numbers
+except ImportError:
+    _atomic_numbers = {s: z for z, s in enumerate(_SYMBOLS)}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def mic(dr, cell, pbc=None):
+    """Apply the minimum image convention to distance vectors ``dr``
+    (neighbours.py:39-62 of the reference)."""
+    dr = np.asarray(dr, dtype=float)
+    cell = np.asarray(cell, dtype=float)
+    rec = np.linalg.inv(cell)
+    if pbc is not None:
+        rec = rec * np.asarray(pbc, dtype=int).reshape(3, 1)
+    offset = np.round(dr @ rec)
+    return dr - offset @ cell
+
+
+class DLPackTensor:
+    """A zero-copy DLPack tensor produced by the neighbour list; implements
+    ``__dlpack__`` / ``__dlpack_device__`` (neighbours.py:65-81). Like the
+    reference it hands out one capsule, so it is single use."""
+
+    __slots__ = ("_capsule", "_device")
+
+    def __init__(self, capsule, device):
+        self._capsule = capsule
+        self._device = device
+
+    def __dlpack__(self, *args, **kwargs):
+        return self._capsule
+
+    def __dlpack_device__(self):
+        return self._device
+
+
+def _dlpack_device(x):
+    fn = getattr(x, "__dlpack_device__", None)
+    if fn is None:
+        return None
+    try:
+        dt, did = fn()
+        return (int(dt), int(did))
+    except Exception:  # pragma: no cover - defensive
+        return None
+
+
+def _is_on_device(x):
+    dev = _dlpack_device(x)
+    return dev is not None and dev[0] != _DLPACK_CPU
+
+
+def _resolve_device(device, on_device_input):
+    """neighbours.py:111-128: None follows the input; "cpu"; "cuda"/"gpu"/
+    "rocm"/"hip"; an int id; or ("cuda", id). Returns (want_device_outputs,
+    device_id)."""
+    if device is None:
+        return (on_device_input, -1)
+    if isinstance(device, str):
+        d = device.lower()
+        if d == "cpu":
+            return (False, -1)
+        if d in ("cuda", "gpu", "rocm", "hip"):
+            return (True, -1)
+        raise ValueError(f"Unknown device {device!r}.")
+    if isinstance(device, int):
+        return (True, device)
+    if isinstance(device, (tuple, list)) and len(device) == 2:
+        return (True, int(device[1]))
+    raise ValueError(f"Unknown device {device!r}.")
+
+
+def _host_metadata(cell, pbc, numbers, cell_origin, nat, *, positions=None):
+    """neighbours.py:145-164: defaults, shrink-wrap, inv_cell = inv(cell.T)."""
+    if cell is None:
+        if positions is None:
+            raise ValueError("cell must be given for device-resident positions.")
+        r = np.asarray(positions, dtype=float)
+        rmin, rmax = r.min(axis=0), r.max(axis=0)
+        cell_origin = rmin if cell_origin is None else cell_origin
+        cell = np.diag(rmax - rmin)
+    if cell_origin is None:
+        cell_origin = np.zeros(3)
+    if pbc is None:
+        pbc = np.zeros(3, dtype=bool)
+    if numbers is None:
+        numbers = np.ones(nat, dtype=np.int32)
+    cell = np.ascontiguousarray(np.asarray(cell, dtype=float))
+    cell_origin = np.ascontiguousarray(np.asarray(cell_origin, dtype=float))
+    pbc = np.ascontiguousarray(np.broadcast_to(pbc, (3,)), dtype=bool)
+    numbers = np.ascontiguousarray(np.asarray(numbers), dtype=np.int32)
+    inv_cell = np.ascontiguousarray(np.linalg.inv(cell.T))
+    return cell_origin, cell, inv_cell, pbc, numbers
+
+
+def _resolve_cutoff(cutoff, numbers):
+    """neighbours.py:167-181: an element-pair dict becomes a symmetric
+    (maxZ+1)^2 matrix; scalars and per-atom arrays pass through."""
+    if isinstance(cutoff, dict):
+        maxnum = int(np.max(numbers))
+        matrix = np.zeros((maxnum + 1, maxnum + 1), dtype=float)
+        for (el1, el2), c in cutoff.items():
+            el1 = _atomic_numbers.get(el1, el1)
+            el2 = _atomic_numbers.get(el2, el2)
+            if el1 <= maxnum and el2 <= maxnum:
+                matrix[el1, el2] = c
+                matrix[el2, el1] = c
+        return matrix, numbers
+    return cutoff, numbers
+
+
+def _parse_cutoff(cutoff, nat):
+    """The C binding's cutoff forms (bind_py_neighbours.cc:130-173): a Python
+    float is the global cutoff; a 1-d array holds per-atom radii (binning
+    cutoff = max 2r, length must be nat: TypeError); a 2-d square array is the
+    per-type matrix (binning cutoff = max, squares used; non-square:
+    TypeError). Like the reference, a Python int is not a scalar: it fails the
+    array conversion with ValueError."""
+    if isinstance(cutoff, float):
+        return float(cutoff), None, None, 0
+    a = np.asarray(cutoff, dtype=np.float64)
+    if a.ndim not in (1, 2):
+        raise ValueError("cutoff must be a float, a 1-d per-atom array or a 2-d "
+                         "per-type array")
+    a = np.ascontiguousarray(a)
+    if a.ndim == 1:
+        if a.shape[0] != nat:
+            raise TypeError("One-dimensional cutoff array must have length that "
+                            "corresponds to position array.")
+        rc = 0.0
+        for v in a:
+            rc = max(rc, 2 * float(v))
+        return rc, a, None, 0
+    if a.shape[0] != a.shape[1]:
+        raise TypeError("Two-dimensional cutoff array must be square.")
+    rc = 0.0
+    for v in a.ravel():
+        rc = max(rc, float(v))
+    return rc, None, np.ascontiguousarray(a.ravel() * a.ravel()), int(a.shape[0])
+
+
+def _quantity_flags(quantities):
+    if not isinstance(quantities, str):
+        raise TypeError("quantities must be a str")
+    flags = 0
+    for q in quantities:
+        if q not in _lib.QUANTITY_BITS:
+            raise ValueError("Unsupported quantity specified.")
+        flags |= _lib.QUANTITY_BITS[q]
+    return flags
+
+
+def _gather(atoms, positions, cell, pbc, numbers, cell_origin):
+    """neighbours.py:184-197 (an ASE Atoms object or explicit arrays)."""
+    if atoms is not None:
+        if any(x is not None for x in (positions, cell, pbc, numbers, cell_origin)):
+            raise ValueError("Cannot combine an ASE Atoms object with explicit "
+                             "positions/cell/pbc/numbers/cell_origin.")
+        return (atoms.positions, np.asarray(atoms.cell), atoms.pbc,
+                np.asarray(atoms.numbers).astype(np.int32), np.zeros(3))
+    if positions is None:
+        raise ValueError("Provide either an ASE Atoms object or a positions array.")
+    return positions, cell, pbc, numbers, cell_origin
+
+
+def _device_positions(positions):
+    """A device array as a torch float64 (n, 3) contiguous tensor (zero copy
+    through DLPack when it is not a torch tensor already)."""
+    torch = _torch()
+    t = positions if isinstance(positions, torch.Tensor) else torch.from_dlpack(positions)
+    if t.dtype != torch.float64 or t.dim() != 2 or t.shape[1] != 3 or not t.is_contiguous():
+        raise TypeError("device positions must be a C-contiguous float64 array of "
+                        "shape (n, 3)")
+    return t
+
+
+class _Build:
+    """One engine build on a device: count (binning + count + scan) then the
+    requested fills, with outputs allocated from torch's caching allocator."""
+
+    def __init__(self, positions_t, co, ce, inv, pbc, cutoff, numbers, backend=-1):
+        torch = _torch()
+        self.torch = torch
+        self.dev = positions_t.device
+        self.nat = int(positions_t.shape[0])
+        rc, per_atom, per_type_sq, nc = _parse_cutoff(cutoff, self.nat)
+        self.rc = rc
+        self.per_atom_t = (None if per_atom is None else
+                           torch.as_tensor(per_atom, dtype=torch.float64, device=self.dev))
+        self.per_type_sq = per_type_sq
+        self.nc = nc
+        self.types_t = None
+        if per_type_sq is not None:
+            self.types_t = torch.as_tensor(np.asarray(numbers, dtype=np.int32),
+                                           dtype=torch.int32, device=self.dev)
+            if self.types_t.shape[0] != self.nat:
+                raise TypeError("Position and type arrays must have identical first "
+                                "dimension.")
+        self.co = np.ascontiguousarray(co, dtype=np.float64)
+        self.ce = np.ascontiguousarray(ce, dtype=np.float64)
+        self.inv = np.ascontiguousarray(inv, dtype=np.float64)
+        self.pbc = np.ascontiguousarray(np.asarray(pbc, dtype=bool), dtype=np.int32)
+        self.pos = positions_t
+        self.backend = backend
+
+    def run(self, flags, want_counts=False, f32=False):
+        torch = self.torch
+        lib = _lib.load()
+        idx = self.dev.index if self.dev.index is not None else torch.cuda.current_device()
+        plan = _lib.plan_for(idx)
+        out = {}
+        with plan.lock, torch.cuda.device(idx):
+            stream = torch.cuda.current_stream(idx).cuda_stream
+            npairs = _lib.ctypes.c_int64(0)
+            _lib.check(lib.mnl_plan_count(
+                plan.handle, _lib.ctypes.c_void_p(stream), _lib.ptr(self.co),
+                _lib.ptr(self.ce), _lib.ptr(self.inv), _lib.ptr(self.pbc), self.nat,
+                _lib.ptr(self.pos), self.rc, _lib.ptr(self.per_atom_t),
+                _lib.ptr(self.per_type_sq), self.nc, _lib.ptr(self.types_t),
+                int(self.backend), _lib.ctypes.byref(npairs)))
+            P = int(npairs.value)
+            fdt = torch.float32 if f32 else torch.float64
+            kw = dict(device=self.dev)
+            if flags & 1:
+                out["i"] = torch.empty(P, dtype=torch.int32, **kw)
+            if flags & 2:
+                out["j"] = torch.empty(P, dtype=torch.int32, **kw)
+            if flags & 4:
+                out["D"] = torch.empty((P, 3), dtype=fdt, **kw)
+            if flags & 8:
+                out["d"] = torch.empty(P, dtype=fdt, **kw)
+            if flags & 16:
+                out["S"] = torch.empty((P, 3), dtype=torch.int32, **kw)
+            if flags:
+                fill = lib.mnl_plan_fill_f32 if f32 else lib.mnl_plan_fill
+                _lib.check(fill(plan.handle, _lib.ctypes.c_void_p(stream),
+                                _lib.ptr(out.get("i")), _lib.ptr(out.get("j")),
+                                _lib.ptr(out.get("D")), _lib.ptr(out.get("d")),
+                                _lib.ptr(out.get("S"))))
+            if want_counts:
+                out["c"] = torch.empty(self.nat, dtype=torch.int32, **kw)
+                _lib.check(lib.mnl_plan_counts(plan.handle, _lib.ctypes.c_void_p(stream),
+                                               _lib.ptr(out["c"])))
+        self.npairs = P
+        return out
+
+
+def _is_f32(out_dtype):
+    if out_dtype is None:
+        return False
+    name = str(getattr(out_dtype, "__name__", out_dtype))
+    if "float32" in name:
+        return True
+    if "float64" in name:
+        return False
+    raise ValueError(f"out_dtype must be float64 or float32, got {out_dtype!r}")
+
+
+def _prepare(atoms, positions, cell, pbc, numbers, cell_origin, device):
+    """Common front half of every entry point: normalise the arguments and put
+    positions on the device. Returns (positions_t, co, ce, inv, pb, nums,
+    on_device, want_device_out)."""
+    positions, cell, pbc, numbers, cell_origin = _gather(
+        atoms, positions, cell, pbc, numbers, cell_origin)
+    on_device = _is_on_device(positions)
+    want_dev, device_id = _resolve_device(device, on_device)
+    if on_device and device is not None and not want_dev:
+        raise ValueError("device='cpu' with device-resident positions is not "
+                         "supported; move the array to the host first.")
+    _lib.require_gpu()
+    torch = _torch()
+    if on_device:
+        pos_t = _device_positions(positions)
+        if device_id >= 0 and pos_t.device.index != device_id:
+            pos_t = pos_t.to(f"cuda:{device_id}")
+        co, ce, inv, pb, nums = _host_metadata(cell, pbc, numbers, cell_origin,
+                                               int(pos_t.shape[0]))
+    else:
+        host_pos = np.ascontiguousarray(positions, dtype=float)
+        if host_pos.ndim != 2 or host_pos.shape[1] != 3:
+            raise ValueError("positions must have shape (n, 3)")
+        co, ce, inv, pb, nums = _host_metadata(cell, pbc, numbers, cell_origin,
+                                               len(host_pos), positions=host_pos)
+        dev = device_id if device_id >= 0 else torch.cuda.current_device()
+        pos_t = torch.from_numpy(host_pos).to(f"cuda:{dev}")
+    return pos_t, co, ce, inv, pb, nums, on_device, want_dev
+
+
+def _export(tensors, want_dev, array_namespace):
+    """Hand results over in the requested form (neighbours.py:131-142)."""
+    if array_namespace == "dlpack":
+        if not want_dev:
+            tensors = [t.cpu() for t in tensors]
+        return [DLPackTensor(t.__dlpack__(), t.__dlpack_device__()) for t in tensors]
+    if array_namespace is None:
+        if want_dev:
+            return list(tensors)
+        return [t.cpu().numpy() for t in tensors]
+    if not want_dev:
+        tensors = [t.cpu() for t in tensors]
+    return [array_namespace.from_dlpack(t) for t in tensors]
+
+
+def neighbour_list(quantities, atoms=None, cutoff=None, *, positions=None, cell=None,
+                   pbc=None, numbers=None, cell_origin=None, device=None,
+                   array_namespace=None, out_dtype=None):
+    """Compute a neighbour list (neighbours.py:200-288 of the reference).
+
+    quantities: any of ``i`` (first index), ``j`` (second index), ``d``
+    (distance), ``D`` (distance vector), ``S`` (cell shift), returned in the
+    order given; a single character returns a bare array. ``cutoff`` is a
+    float, an element-pair dict or a per-atom radius array. Pairs are sorted by
+    ``i`` in the reference's order and ``D == r[j] - r[i] + S @ cell``.
+    """
+    if cutoff is None:
+        raise ValueError("Please provide a value for the cutoff radius.")
+    flags = _quantity_flags(quantities)
+    f32 = _is_f32(out_dtype)
+    pos_t, co, ce, inv, pb, nums, on_device, want_dev = _prepare(
+        atoms, positions, cell, pbc, numbers, cell_origin, device)
+    rc, nums = _resolve_cutoff(cutoff, nums)
+    b = _Build(pos_t, co, ce, inv, pb, rc, nums)
+    out = b.run(flags, f32=f32)
+    tensors = [out[q] for q in quantities]
+    arrays = _export(tensors, want_dev, array_namespace)
+    return arrays[0] if len(quantities) == 1 else tuple(arrays)
+
+
+def coordination(atoms=None, cutoff=None, *, positions=None, cell=None, pbc=None,
+                 numbers=None, cell_origin=None, device=None, array_namespace=None):
+    """Neighbours per atom within ``cutoff`` (neighbours.py:373-422), from the
+    count pass only: the pair list is never materialised."""
+    if cutoff is None:
+        raise ValueError("Please provide a value for the cutoff radius.")
+    pos_t, co, ce, inv, pb, nums, on_device, want_dev = _prepare(
+        atoms, positions, cell, pbc, numbers, cell_origin, device)
+    rc, nums = _resolve_cutoff(cutoff, nums)
+    b = _Build(pos_t, co, ce, inv, pb, rc, nums)
+    out = b.run(0, want_counts=True)
+    counts = out["c"]
+    if not want_dev and array_namespace is None:
+        return counts.cpu().numpy().astype(np.int64)
+    return _export([counts], want_dev, array_namespace)[0]
+
+
+def first_neighbours(n, i):
+    """Row-start index of each atom in an i-sorted pair list
+    (first_neighbours.cc:15-40); -1 for atoms before the first populated row."""
+    i = np.ascontiguousarray(i, dtype=np.int32)
+    n = int(n)
+    seed = np.full(n + 1, -1, dtype=np.int32)
+    nn = i.shape[0]
+    if nn == 0:
+        seed[:] = 0
+        return seed
+    seed[i[0]] = 0
+    jumps = np.nonzero(i[1:] != i[:-1])[0] + 1
+    for k in jumps:
+        seed[i[k - 1] + 1:i[k] + 1] = k
+    seed[i[-1] + 1:] = nn
+    return seed
+
+
+def get_jump_indicies(sorted_array):
+    """Positions where a sorted array changes value, plus both ends
+    (first_neighbours.cc:42-55)."""
+    s = np.ascontiguousarray(sorted_array, dtype=np.int32)
+    nn = s.shape[0]
+    n = int(np.count_nonzero(s[1:] != s[:-1])) + 1 if nn > 0 else 1
+    return first_neighbours(n, s)
+
+
+def triplet_list(first_neighbours, abs_dr_p=None, cutoff=None):
+    """Triplets (ij, ik) of pairs sharing the first atom (triplet_list.cc:15-34,
+    neighbours.py:360-370)."""
+    fi = np.ascontiguousarray(first_neighbours, dtype=np.int32)
+    if (abs_dr_p is None) != (cutoff is None):
+        raise TypeError("Cutoff and distances must be specified together.")
+    if abs_dr_p is not None and not isinstance(cutoff, float):
+        raise NotImplementedError("Cutoff must be a single float.")
+    d = None if abs_dr_p is None else np.ascontiguousarray(abs_dr_p, dtype=float)
+    ij_t, ik_t = [], []
+    for r in range(fi.shape[0] - 1):
+        rows = np.arange(fi[r], fi[r + 1], dtype=np.int32)
+        if rows.size == 0:
+            continue
+        a = np.repeat(rows, rows.size)
+        b = np.tile(rows, rows.size)
+        keep = a != b
+        if d is not None:
+            keep &= (d[a] < cutoff) & (d[b] < cutoff)
+        ij_t.append(a[keep])
+        ik_t.append(b[keep])
+    if not ij_t:
+        return np.zeros(0, np.int32), np.zeros(0, np.int32)
+    return np.concatenate(ij_t).astype(np.int32), np.concatenate(ik_t).astype(np.int32)
+
+
+_lock = threading.Lock()

@@ -1,0 +1,135 @@
+"""The C ABI on the GPU: one-shot entry points, host copies, DLPack export,
+count-only, error reporting, and the device square root."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import assert_same_pairs, bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _lib():
+    from matscipy_neighbours_amd import _lib as L
+    return L.load()
+
+
+def _geom(cell, pbc):
+    cell = np.ascontiguousarray(cell, dtype=np.float64)
+    inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+    pb = np.ascontiguousarray(np.broadcast_to(np.asarray(pbc, bool), (3,)), dtype=np.int32)
+    return np.zeros(3), cell, inv, pb
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def test_device_sqrt_matches_host(gpu):
+    """d is sqrt(r2) on the device; it must round exactly like the host's
+    IEEE sqrt. Random magnitudes, squares of representable values, and
+    subnormal/huge edge values."""
+    rng = np.random.default_rng(0)
+    x = np.concatenate([
+        rng.uniform(0, 50, 200000),
+        np.exp(rng.uniform(-700, 700, 100000)),
+        rng.uniform(0, 1, 1000) ** 2,
+        np.array([0.0, 1.0, 2.0, 4.0, 1e-310, 5e-324, 1.7e308, 25.0, 24.999999999999996]),
+        np.nextafter(np.arange(1, 1000, dtype=float) ** 2, 0),
+    ])
+    y = np.empty_like(x)
+    assert _lib().mnl_debug_sqrt(_p(x), _p(y), x.size) == 0
+    assert np.array_equal(bits(y), bits(np.sqrt(x)))
+
+
+def test_one_shot_host_in_host_out(gpu):
+    L = _lib()
+    rng = np.random.default_rng(3)
+    pos = np.ascontiguousarray(rng.uniform(0, 12, (2000, 3)))
+    co, cell, inv, pb = _geom(12 * np.eye(3), True)
+    res = ctypes.c_void_p()
+    st = L.mnl_neighbour_list_device(31, _p(co), _p(cell), _p(inv), _p(pb), 2000, _p(pos), 0,
+                                     1.0, None, None, 0, None, -1, ctypes.byref(res))
+    assert st == 0, L.mnl_last_error()
+    try:
+        P = L.mnl_result_npairs(res)
+        ref = oracle.neighbour_list("ijdDS", co, cell, inv, pb, pos, 1.0)
+        assert P == len(ref["i"])
+        got = {"i": np.empty(P, np.int32), "j": np.empty(P, np.int32),
+               "d": np.empty(P), "D": np.empty((P, 3)), "S": np.empty((P, 3), np.int32)}
+        for q, a in got.items():
+            assert L.mnl_result_copy_to_host(res, q.encode(), _p(a)) == 0
+        assert_same_pairs(got, ref)
+        assert L.mnl_result_nat(res) == 2000
+    finally:
+        L.mnl_result_free(res)
+
+
+def test_dlpack_export_zero_copy(gpu):
+    """Export through DLPack and adopt in torch without a copy."""
+    import torch
+    L = _lib()
+    rng = np.random.default_rng(4)
+    pos = np.ascontiguousarray(rng.uniform(0, 10, (1000, 3)))
+    co, cell, inv, pb = _geom(10 * np.eye(3), True)
+    res = ctypes.c_void_p()
+    assert L.mnl_neighbour_list_device(31, _p(co), _p(cell), _p(inv), _p(pb), 1000, _p(pos),
+                                       0, 2.5, None, None, 0, None, -1,
+                                       ctypes.byref(res)) == 0
+    ref = oracle.neighbour_list("ijdDS", co, cell, inv, pb, pos, 2.5)
+    PyCapsule_New = ctypes.pythonapi.PyCapsule_New
+    PyCapsule_New.restype = ctypes.py_object
+    PyCapsule_New.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+    try:
+        for q in "ijDdS":
+            dev_ptr = L.mnl_result_data(res, q.encode())
+            t_ptr = L.mnl_result_export_dlpack(res, q.encode())
+            assert t_ptr, L.mnl_last_error()
+            assert L.mnl_result_data(res, q.encode()) is None
+            cap = PyCapsule_New(t_ptr, b"dltensor", None)
+            t = torch.from_dlpack(cap)
+            assert t.is_cuda and t.data_ptr() == dev_ptr
+            assert np.array_equal(bits(t.cpu().numpy()), bits(ref[q]))
+            del t
+        assert not L.mnl_result_export_dlpack(res, b"i")
+    finally:
+        L.mnl_result_free(res)
+
+
+def test_count_only(gpu):
+    L = _lib()
+    rng = np.random.default_rng(5)
+    pos = np.ascontiguousarray(rng.uniform(0, 14, (3000, 3)))
+    co, cell, inv, pb = _geom(14 * np.eye(3), True)
+    res = ctypes.c_void_p()
+    assert L.mnl_neighbour_count_device(_p(co), _p(cell), _p(inv), _p(pb), 3000, _p(pos), 0,
+                                        1.2, None, None, 0, None, -1,
+                                        ctypes.byref(res)) == 0
+    try:
+        c = np.empty(3000, np.int32)
+        assert L.mnl_result_copy_to_host(res, b"c", _p(c)) == 0
+        ref = oracle.neighbour_list("i", co, cell, inv, pb, pos, 1.2, with_counts=True)
+        assert np.array_equal(c, ref["counts"])
+        assert L.mnl_result_npairs(res) == len(ref["i"])
+        assert L.mnl_result_data(res, b"i") is None
+    finally:
+        L.mnl_result_free(res)
+
+
+def test_zero_volume_error(gpu):
+    """tests/test_core.cc:64-73: NL_ERROR and a message, not a crash."""
+    L = _lib()
+    bad = np.ascontiguousarray([[1.0, 0, 0], [2.0, 0, 0], [0, 0, 1.0]])
+    inv = np.ascontiguousarray(np.eye(3))
+    pb = np.ones(3, np.int32)
+    pos = np.array([[0.5, 0.5, 0.5]])
+    res = ctypes.c_void_p()
+    st = L.mnl_neighbour_list_device(1, _p(np.zeros(3)), _p(bad), _p(inv), _p(pb), 1,
+                                     _p(pos), 0, 1.0, None, None, 0, None, -1,
+                                     ctypes.byref(res))
+    assert st == -1
+    assert L.mnl_last_error() == b"Zero cell volume."
+    assert not res.value

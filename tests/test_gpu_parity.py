@@ -1,0 +1,237 @@
+"""HIP engine vs the oracle: bit-exact pairs, order, D, d and S.
+
+Every case runs the engine through the C ABI (libmnl.so via the Python
+binding) on the GPU and the oracle (the CPU restatement of the reference
+path) on the same inputs, with the same inv_cell bytes, and requires
+identical arrays bit for bit. Cases follow the reference's own tests
+(tests/test_extension.py, tests/test_core.cc, tests/test_neighbour_list_gpu.cc)
+plus boundary stress (atoms on cell faces, pairs exactly at the cutoff),
+tiny cells (multi-image shifts), both grid backends and random triclinic
+fuzz.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import assert_same_pairs
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(quantities, positions, cell, pbc, cutoff, numbers=None, backend=-1,
+            f32=False):
+    import torch
+    from matscipy_neighbours_amd import neighbours as nb
+    pos = np.ascontiguousarray(positions, dtype=float)
+    co, ce, inv, pb, nums = nb._host_metadata(cell, pbc, numbers, None, len(pos),
+                                              positions=pos)
+    rc, nums = nb._resolve_cutoff(cutoff, nums)
+    b = nb._Build(torch.from_numpy(pos).to("cuda:0"), co, ce, inv, pb, rc, nums,
+                  backend=backend)
+    out = b.run(nb._quantity_flags(quantities), f32=f32, want_counts=True)
+    torch.cuda.synchronize()
+    return {k: v.cpu().numpy() for k, v in out.items()}
+
+
+def _oracle(quantities, positions, cell, pbc, cutoff, numbers=None):
+    pos = np.ascontiguousarray(positions, dtype=float)
+    cell = np.asarray(cell, dtype=float)
+    inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+    types = None
+    if numbers is not None and np.asarray(cutoff).ndim == 2:
+        types = numbers
+    return oracle.neighbour_list(quantities, np.zeros(3), cell, inv, pbc, pos, cutoff,
+                                 types=types, with_counts=True)
+
+
+def _check(positions, cell, pbc, cutoff, numbers=None, backends=(-1,), msg=""):
+    ref = _oracle("ijdDS", positions, cell, pbc, cutoff, numbers)
+    for be in backends:
+        got = _engine("ijdDS", positions, cell, pbc, cutoff, numbers, backend=be)
+        assert_same_pairs(got, ref, msg=f"{msg} backend={be}")
+        assert np.array_equal(got["c"], ref["counts"]), f"{msg} counts"
+    return ref
+
+
+# ------------------------------------------------------------ known answers
+
+def test_unit_cube_known_answers(gpu):
+    """tests/test_core.cc:32-62, tests/test_extension.py:102-119."""
+    pos = np.array([[0.5, 0.5, 0.5]])
+    ref = _check(pos, np.eye(3), True, 1.1)
+    assert len(ref["i"]) == 6
+    assert np.array_equal(ref["D"], ref["S"].astype(float))
+    assert len(_check(pos, np.eye(3), True, 1.5)["i"]) == 18
+    assert len(_check(pos, np.eye(3), False, 1.1)["i"]) == 0
+    assert len(_check(pos, np.eye(3), [True, False, False], 1.1)["i"]) == 2
+    assert len(_check(pos, np.eye(3), [True, False, True], 1.1)["i"]) == 4
+
+
+def test_simple_cubic_2x2x2(gpu):
+    """tests/test_core.cc:75-122: 8 atoms, 48 pairs."""
+    g = np.array([[x, y, z] for x in range(2) for y in range(2) for z in range(2)], float)
+    ref = _check(g + 0.5, 2 * np.eye(3), True, 1.1)
+    assert len(ref["i"]) == 48
+
+
+def test_per_atom_and_per_type(gpu):
+    """tests/test_extension.py:211-231."""
+    cell = 10.0 * np.eye(3)
+    pos = np.array([[0.0, 0, 0], [1.0, 0, 0], [2.5, 0, 0]])
+    ref = _check(pos, cell, False, np.array([0.4, 0.4, 1.2]))
+    assert sorted(zip(ref["i"].tolist(), ref["j"].tolist())) == [(1, 2), (2, 1)]
+    pos = np.array([[0.0, 0, 0], [1.0, 0, 0], [2.0, 0, 0]])
+    types = np.array([0, 1, 0], dtype=np.int32)
+    ref = _check(pos, cell, False, np.array([[0.5, 1.5], [1.5, 0.5]]), numbers=types)
+    assert sorted(zip(ref["i"].tolist(), ref["j"].tolist())) == [(0, 1), (1, 0), (1, 2),
+                                                                  (2, 1)]
+
+
+def test_per_type_out_of_range_types_keep_global_test(gpu):
+    """Types outside the matrix fall back to the global cutoff
+    (neighbour_visit.hh:100-108)."""
+    rng = np.random.default_rng(11)
+    pos = rng.uniform(0, 8, (300, 3))
+    types = rng.integers(-1, 4, 300).astype(np.int32)
+    _check(pos, 8 * np.eye(3), True, np.array([[1.0, 1.5, 1.2], [1.5, 2.0, 0.9],
+                                               [1.2, 0.9, 1.7]]), numbers=types)
+
+
+# ------------------------------------------------------------ reference scenarios
+
+@pytest.mark.parametrize("pbc", [True, False, (True, False, True), (False, True, False)])
+def test_cubic_random(gpu, pbc):
+    """tests/test_extension.py:126-134."""
+    rng = np.random.default_rng(1234)
+    _check(rng.uniform(0, 5.0, (40, 3)), 5.0 * np.eye(3), pbc, 1.7, backends=(0, 1))
+
+
+@pytest.mark.parametrize("pbc", [True, False, (True, True, False)])
+def test_triclinic_atoms_outside(gpu, pbc):
+    """tests/test_extension.py:137-148."""
+    rng = np.random.default_rng(7)
+    cell = np.array([[4.0, 0.0, 0.0], [1.3, 3.7, 0.0], [0.8, 0.6, 4.2]])
+    pos = rng.uniform(-0.3, 1.3, (30, 3)) @ cell
+    _check(pos, cell, pbc, 1.6, backends=(0, 1))
+
+
+def test_atoms_far_outside(gpu):
+    """tests/test_extension.py:163-174, including the non-periodic shift quirk
+    for atoms more than a box length outside."""
+    rng = np.random.default_rng(99)
+    cell = 6.0 * np.eye(3)
+    pos = rng.uniform(0, 6.0, (25, 3))
+    pos[3] += cell[0]
+    pos[7] -= 2 * cell[1]
+    pos[11] += 3 * cell[2]
+    for pbc in (True, False, (True, False, True)):
+        _check(pos, cell, pbc, 1.5, backends=(0, 1), msg=str(pbc))
+
+
+@pytest.mark.parametrize("pbc", [True, False, (True, False, True)])
+def test_vacuum_box(gpu, pbc):
+    """tests/test_extension.py:151-160 (dense by the reference rule; also
+    forced through the hashed grid)."""
+    rng = np.random.default_rng(3)
+    _check(rng.uniform(0, 6.0, (250, 3)), 120.0 * np.eye(3), pbc, 1.5, backends=(-1, 0, 1))
+
+
+@pytest.mark.parametrize("n,L,rc,seed", [(2000, 12.0, 1.0, 1), (500, 3.5, 1.0, 2),
+                                         (3000, 15.0, 2.0, 3)])
+def test_gpu_suite_scenarios(gpu, n, L, rc, seed):
+    """tests/test_neighbour_list_gpu.cc:95-110 (there compared as multisets;
+    here bit for bit, order included)."""
+    rng = np.random.default_rng(seed)
+    _check(rng.uniform(0, L, (n, 3)), L * np.eye(3), True, rc, backends=(0, 1))
+
+
+def test_sparse_vacuum_hashed(gpu):
+    """tests/test_neighbour_list_gpu.cc:120-127: 4000 atoms in a box 40x
+    larger than the cluster: the reference rule selects the hashed grid."""
+    rng = np.random.default_rng(11)
+    Lc = (4000 / 12.0) ** (1 / 3)
+    ref = _oracle("ijdDS", rng.uniform(0, Lc, (4000, 3)), 40 * Lc * np.eye(3), True, 1.0)
+    assert ref["_sparse"]
+    rng = np.random.default_rng(11)
+    _check(rng.uniform(0, Lc, (4000, 3)), 40 * Lc * np.eye(3), True, 1.0, backends=(-1, 0))
+
+
+def test_cfg1_uniform_1000(gpu):
+    """1000 atoms uniform in a 10 A periodic cube, rc 2.5 (~65k pairs)."""
+    from matscipy_neighbours_amd import workloads
+    w = workloads.uniform_box()
+    ref = _check(w["positions"], w["cell"], w["pbc"], w["cutoff"], backends=(0, 1))
+    assert 60000 < len(ref["i"]) < 70000
+
+
+# ------------------------------------------------------------ boundary stress
+
+@pytest.mark.parametrize("rc", [1.0, np.sqrt(2.0), np.sqrt(3.0), 2.0])
+def test_lattice_at_shell_distance(gpu, rc):
+    """Simple-cubic lattice with rc exactly at a shell distance: every pair at
+    r2 == rc^2 must be excluded by the same rounding as the reference."""
+    g = np.arange(6, dtype=float)
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    _check(pos, 6 * np.eye(3), True, float(rc), backends=(0, 1))
+
+
+def test_atoms_on_cell_faces(gpu):
+    """Atoms exactly on bin faces k*bin (floor of the cell index decides)."""
+    cell = 8.0 * np.eye(3)
+    g = np.arange(0, 8.0, 2.0)
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pos = np.concatenate([pos, pos + 1.0, pos - 8.0, pos + 8.0])
+    for pbc in (True, False, (False, True, True)):
+        _check(pos, cell, pbc, 2.0, backends=(0, 1), msg=str(pbc))
+
+
+@pytest.mark.parametrize("cut", [0.7, 1.5, 2.5, 3.3])
+def test_tiny_cells_multi_image(gpu, cut):
+    """Boxes narrower than the cutoff: n = 1, several images per neighbour
+    and |S| >= 2 (nx = ceil(rc / width))."""
+    rng = np.random.default_rng(21)
+    cell = np.array([[1.0, 0.0, 0.0], [0.2, 1.1, 0.0], [0.1, -0.3, 0.9]])
+    pos = rng.uniform(0, 1, (5, 3)) @ cell
+    for pbc in (True, (True, True, False)):
+        ref = _check(pos, cell, pbc, cut, backends=(0, 1), msg=str(pbc))
+        if cut > 2.0 and pbc is True:
+            assert np.abs(ref["S"]).max() >= 2
+
+
+def test_empty_and_single(gpu):
+    _check(np.zeros((0, 3)), 5 * np.eye(3), True, 1.0)
+    _check(np.array([[1.0, 2.0, 3.0]]), 5 * np.eye(3), True, 1.0)
+
+
+def test_big_cell_occupancy(gpu):
+    """More than 64 atoms per cell (home-atom groups of 64)."""
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(0, 4.0, (900, 3))
+    _check(pos, 4.0 * np.eye(3), True, 1.9, backends=(0, 1))
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_triclinic_fuzz(gpu, seed):
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.integers(1, 400))
+    cell = np.diag(rng.uniform(2.0, 12.0, 3)) + rng.uniform(-1.5, 1.5, (3, 3)) * (seed % 3 > 0)
+    pbc = rng.integers(0, 2, 3).astype(bool)
+    pos = rng.uniform(-0.4, 1.4, (n, 3)) @ cell
+    cutoff = float(rng.uniform(0.3, 4.0))
+    _check(pos, cell, pbc, cutoff, backends=(0, 1), msg=f"seed={seed}")
+
+
+def test_fp32_output_option(gpu):
+    """float32 D and d: same pairs as float64, values within 2e-6 relative."""
+    from matscipy_neighbours_amd import workloads
+    w = workloads.uniform_box(seed=9)
+    ref = _oracle("ijdDS", w["positions"], w["cell"], w["pbc"], w["cutoff"])
+    got = _engine("ijdDS", w["positions"], w["cell"], w["pbc"], w["cutoff"], f32=True)
+    for q in "ijS":
+        assert np.array_equal(got[q], ref[q])
+    assert got["D"].dtype == np.float32 and got["d"].dtype == np.float32
+    # tolerance: float32 rounding of a float64 value, 2e-6 relative to |D|
+    np.testing.assert_allclose(got["d"], ref["d"], rtol=2e-6, atol=0)
+    scale = np.maximum(np.abs(ref["D"]), ref["d"][:, None])
+    assert np.all(np.abs(got["D"] - ref["D"]) <= 2e-6 * scale)
