@@ -72,7 +72,20 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--reps", type=int, default=64, help="fcc unit cells per edge")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device-to-device copy (to scale HBM "
+                         "traffic counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(max(args.steps, 1)):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read_per_launch": a.numel() * 8,
+                          "bytes_written_per_launch": b.numel() * 8}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(the neighbour-list engine runs on a single device; the multi-GPU "

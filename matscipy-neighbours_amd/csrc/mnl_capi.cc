@@ -122,7 +122,7 @@ struct mnl_plan {
     /* workspaces */
     DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, counts,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
-        htypes;
+        htypes, masks;
     int64_t *host_total = nullptr; /* pinned */
     /* state of the last count */
     bool counted = false;
@@ -159,6 +159,8 @@ PairArgs pair_args(const mnl_plan *p) {
     a.counts = p->counts.as<int32_t>();
     a.offsets = p->offsets.as<int64_t>();
     a.capacity = p->npairs;
+    a.masks = p->masks.as<uint64_t>();
+    a.mask_stride = p->nat;
     return a;
 }
 
@@ -235,6 +237,7 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
                      ? sizeof(double) * std::max<int64_t>(1, int64_t(ncutoffs) * ncutoffs)
                      : 0},
         {&p->counts, n * sizeof(int32_t)},
+        {&p->masks, n * kMaskChunksHost * sizeof(uint64_t)},
         {&p->offsets, (n + 1) * sizeof(int64_t)},
         {&p->cell_count, static_cast<size_t>(cells_or_slots) * sizeof(int32_t)},
         {&p->cell_first, static_cast<size_t>(cells_or_slots) * sizeof(int32_t)},
@@ -539,7 +542,8 @@ void mnl_plan_destroy(mnl_plan *p) {
     DevBuf *bufs[] = {&p->raw4,      &p->ticket,    &p->tmp,    &p->spos,     &p->sinfo,
                       &p->stype,     &p->cell_count, &p->cell_first, &p->counts, &p->offsets,
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
-                      &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes};
+                      &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
+                      &p->masks};
     for (DevBuf *b : bufs) b->release();
     if (p->host_total) (void)hipHostFree(p->host_total);
     delete p;

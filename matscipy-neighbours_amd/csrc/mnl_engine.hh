@@ -75,6 +75,8 @@ struct PairArgs {
     int32_t *counts;           /* count: [N] by global atom index */
     const int64_t *offsets;    /* fill: [N+1] */
     int64_t capacity;          /* fill: pairs beyond this are not written */
+    uint64_t *masks;           /* [kMaskChunks][mask_stride] accept masks */
+    int64_t mask_stride;       /* total atoms */
     int32_t *out_i;
     int32_t *out_j;
     void *out_D;
@@ -112,6 +114,8 @@ hipError_t launch_scan_i32_to_i32(const int32_t *in, int64_t n, int32_t *out,
 hipError_t launch_scan_i32_to_i64(const int32_t *in, int64_t n, int64_t *out,
                                   int64_t *partial, int64_t *total,
                                   hipStream_t stream);
+/* Accept-mask chunks kept per atom by the count pass (see mnl_kernels.hip). */
+constexpr int kMaskChunksHost = 16;
 /* Pair kernels. mode 0 = count, 1 = fill f64 outputs, 2 = fill f32 outputs. */
 hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
                         hipStream_t stream);

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "mnl_engine.hh"
 
@@ -390,225 +391,510 @@ __device__ __forceinline__ void hashed_slice(const PairArgs &A, int n0, int n1, 
     }
 }
 
+/* Read-only views in the constant address space: a load through them at a
+   wave-uniform address becomes a scalar (SMEM) load into SGPRs. The pair
+   kernels never write these arrays. */
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T *p) {
+    return (cptr<T>)(p);
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+/* Element i of a read-only record array through the constant address space
+   (component-wise: the vector types have no address-space-qualified copy). */
+__device__ __forceinline__ double4 load_const(const double4 *p, int i) {
+    cptr<double> q = (cptr<double>)(p) + 4 * static_cast<int64_t>(i);
+    return make_double4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ int4 load_const(const int4 *p, int i) {
+    cptr<int> q = (cptr<int>)(p) + 4 * static_cast<int64_t>(i);
+    return make_int4(q[0], q[1], q[2], q[3]);
+}
+
+__device__ __forceinline__ int64_t readlane64(int64_t v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(v), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane);
+    return (static_cast<int64_t>(hi) << 32) | static_cast<uint32_t>(lo);
+}
+
+/* Chunks of 64 candidates whose accept masks the count pass keeps for the
+   fill (per home atom); a home cell with more chunks re-tests the rest in
+   the fill. */
+constexpr int kMaskChunks = 16;
+static_assert(kMaskChunks == kMaskChunksHost, "mask buffer sized by the host");
+
+}  // namespace
+/* The v_writelane intrinsic (no clang builtin spells it): returns `old` with
+   lane `lane` replaced by the wave-uniform `val`. */
+extern "C" __device__ int mnl_llvm_writelane(int val, int lane, int old) __asm(
+    "llvm.amdgcn.writelane.i32");
+namespace {
+
+__device__ __forceinline__ uint32_t write_lane(uint32_t v, uint32_t s, int lane) {
+    return static_cast<uint32_t>(
+        mnl_llvm_writelane(static_cast<int>(s), lane, static_cast<int>(v)));
+}
+
+/* Home cell decoded once per wave (all values wave-uniform). */
+struct HomeCell {
+    int m, hb;             /* atoms in the cell, first cell-sorted atom */
+    int c0, c1, c2;        /* cell coordinates */
+    int n0, n1, n2;
+    int nx, ny, nz, sx, sy, nseg, centre;
+    int p0, p1, p2;
+    int abase, cbase;
+    double cutoff_sq;
+    double b00, b01, b02, b10, b11, b12, b20, b21, b22;
+};
+
+template <bool HASHED>
+__device__ __forceinline__ bool load_home(const PairArgs &A, int h, HomeCell &H) {
+    H.m = uniform(as_const(A.cell_count)[h]);
+    if (H.m == 0) return false;
+    H.hb = uniform(as_const(A.cell_first)[h]);
+    int sys = 0;
+    if (!HASHED) sys = system_of(A.cell_base, A.nsys, h);
+    sys = uniform(sys);
+    cptr<SysGeom> gp = as_const(A.geo) + sys;
+    H.n0 = gp->n[0];
+    H.n1 = gp->n[1];
+    H.n2 = gp->n[2];
+    if (!HASHED) {
+        const int lin = h - gp->cell_base;
+        H.c0 = lin % H.n0;
+        H.c1 = (lin / H.n0) % H.n1;
+        H.c2 = lin / (H.n0 * H.n1);
+    } else {
+        const int64_t key = as_const(A.hkey)[h];
+        H.c0 = static_cast<int>(key % H.n0);
+        H.c1 = static_cast<int>((key / H.n0) % H.n1);
+        H.c2 = static_cast<int>(key / (static_cast<int64_t>(H.n0) * H.n1));
+    }
+    H.nx = gp->nxyz[0];
+    H.ny = gp->nxyz[1];
+    H.nz = gp->nxyz[2];
+    H.sx = 2 * H.nx + 1;
+    H.sy = 2 * H.ny + 1;
+    H.nseg = gp->nseg;
+    H.centre = (H.nz * H.sy + H.ny) * H.sx + H.nx; /* segment of offset (0,0,0) */
+    H.p0 = gp->pbc[0];
+    H.p1 = gp->pbc[1];
+    H.p2 = gp->pbc[2];
+    H.cutoff_sq = gp->cutoff_sq;
+    H.b00 = gp->bin[0];
+    H.b01 = gp->bin[1];
+    H.b02 = gp->bin[2];
+    H.b10 = gp->bin[3];
+    H.b11 = gp->bin[4];
+    H.b12 = gp->bin[5];
+    H.b20 = gp->bin[6];
+    H.b21 = gp->bin[7];
+    H.b22 = gp->bin[8];
+    H.abase = gp->atom_base;
+    H.cbase = gp->cell_base;
+    return true;
+}
+
+/* Segment table of one group of up to 64 neighbour-cell offsets, in the
+   reference's traversal order. */
+struct SegLds {
+    double so0[kWave], so1[kWave], so2[kWave]; /* cell offset vector */
+    int st[kWave];    /* first candidate index of each segment */
+    int sb[kWave];    /* first cell-sorted atom of each segment */
+    int swr[kWave];   /* packed periodic wrap counts */
+};
+
+/* Builds the table of segments sg .. sg+nsg-1; returns the group's candidate
+   count, and in *self_start the candidate index of the home cell's first atom
+   at zero offset (-1 if that segment is not in the group). Each segment's
+   offset vector is off = (z b3 + y b2) + x b1
+   (neighbour_visit.hh:67,73-74,80-81). */
+template <bool HASHED>
+__device__ __forceinline__ int build_segments(const PairArgs &A, const HomeCell &H, int sg,
+                                              int nsg, int lane, SegLds &S, int *self_start) {
+    int len = 0, begin = 0, pw = 0;
+    double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+    if (lane < nsg) {
+        const int seg = sg + lane;
+        const int x = seg % H.sx - H.nx;
+        const int rest = seg / H.sx;
+        const int y = rest % H.sy - H.ny;
+        const int z = rest / H.sy - H.nz;
+        int cj0 = H.c0 + x, cj1 = H.c1 + y, cj2 = H.c2 + z;
+        int w0 = 0, w1 = 0, w2 = 0;
+        bool ok = true;
+        if (H.p0) {
+            w0 = floor_div(cj0, H.n0);
+            cj0 -= w0 * H.n0;
+        } else {
+            ok = ok && cj0 >= 0 && cj0 < H.n0;
+        }
+        if (H.p1) {
+            w1 = floor_div(cj1, H.n1);
+            cj1 -= w1 * H.n1;
+        } else {
+            ok = ok && cj1 >= 0 && cj1 < H.n1;
+        }
+        if (H.p2) {
+            w2 = floor_div(cj2, H.n2);
+            cj2 -= w2 * H.n2;
+        } else {
+            ok = ok && cj2 >= 0 && cj2 < H.n2;
+        }
+        if (ok) {
+            if (HASHED)
+                hashed_slice(A, H.n0, H.n1, cj0, cj1, cj2, begin, len);
+            else
+                dense_slice(A, H.cbase, H.n0, H.n1, cj0, cj1, cj2, begin, len);
+        }
+        o0 = (z * H.b20 + y * H.b10) + x * H.b00;
+        o1 = (z * H.b21 + y * H.b11) + x * H.b01;
+        o2 = (z * H.b22 + y * H.b12) + x * H.b02;
+        pw = pack3(w0, w1, w2);
+    }
+    const int incl = wave_incl_scan(len);
+    const int total = uniform(__shfl(incl, kWave - 1, kWave));
+    const int ci = H.centre - sg;
+    *self_start = (ci >= 0 && ci < nsg) ? uniform(__shfl(incl - len, ci, kWave)) : -1;
+    S.so0[lane] = o0;
+    S.so1[lane] = o1;
+    S.so2[lane] = o2;
+    S.st[lane] = incl - len;
+    S.sb[lane] = begin;
+    S.swr[lane] = pw;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return total;
+}
+
+/* Largest index p < n with v[p] <= x, for non-decreasing v with
+   v[0] <= x. Binary lifting with the probe clamped to n - 1, so every LDS
+   read is unconditional (no divergent branches). */
+__device__ __forceinline__ int last_at_or_below(const int *v, int n, int x) {
+    int p = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        const int q = min(p + step, n - 1);
+        p = (v[q] <= x) ? q : p;
+    }
+    return p;
+}
+
+/* Segment holding candidate c: the last one starting at or before c (empty
+   segments never win: the covering one comes later in order). */
+__device__ __forceinline__ int find_segment(const SegLds &S, int nsg, int c) {
+    return last_at_or_below(S.st, nsg, c);
+}
+
+/* One lane's candidate of the current chunk, as the test needs it. */
+struct Cand {
+    double4 pj;            /* bin-relative position (x = NaN for an idle lane) */
+    double o0, o1, o2;     /* cell offset vector */
+    int tj;                /* type (per-type cutoffs) */
+};
+
+template <int CUT>
+__device__ __forceinline__ void load_candidate(const PairArgs &A, const HomeCell &H,
+                                               const SegLds &S, int nsg, int c, int total,
+                                               Cand &C) {
+    const bool cv = c < total;
+    const int p = find_segment(S, nsg, c);
+    const int sj = cv ? S.sb[p] + (c - S.st[p]) : H.hb;
+    C.o0 = S.so0[p];
+    C.o1 = S.so1[p];
+    C.o2 = S.so2[p];
+    C.pj = A.spos[sj];
+    if (!cv) C.pj.x = __builtin_nan("");
+    C.tj = CUT == CUT_PER_TYPE ? A.stype[sj] : 0;
+}
+
+/* The distance test of neighbour_visit.hh:89-109 for home atom si at pi:
+   dr = (rel_j - rel_i) + off, r2 = (dx^2 + dy^2) + dz^2, strict r2 < rc^2,
+   then the per-atom (sum of radii) or per-type (matrix entry when both types
+   are in range) cutoff. The self pair is removed from the ballot by the
+   caller. */
+template <int CUT>
+__device__ __forceinline__ bool pair_test(const PairArgs &A, const HomeCell &H, const Cand &C,
+                                          int si, const double4 &pi) {
+    const double d0 = (C.pj.x - pi.x) + C.o0;
+    const double d1 = (C.pj.y - pi.y) + C.o1;
+    const double d2 = (C.pj.z - pi.z) + C.o2;
+    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    bool ok = r2 < H.cutoff_sq;
+    if (CUT == CUT_PER_ATOM) {
+        const double cc = pi.w + C.pj.w;
+        ok = ok & (r2 < cc * cc);
+    } else if (CUT == CUT_PER_TYPE) {
+        const int ti = as_const(A.stype)[si];
+        if (ti >= 0 && ti < A.ncut) {
+            const bool tj_in = (C.tj >= 0) & (C.tj < A.ncut);
+            const double lim = A.pt_sq[ti * A.ncut + (tj_in ? C.tj : 0)];
+            ok = ok & (!tj_in | (r2 < lim));
+        }
+    }
+    return ok;
+}
+
+/* Ballot of the test of the chunk starting at cb against home atom t of the
+   group, self pair removed: the home atom sits at candidate
+   self_start + grp0 + t of the zero-offset segment (neighbour_visit.hh:86). */
+template <int CUT>
+__device__ __forceinline__ uint64_t chunk_mask(const PairArgs &A, const HomeCell &H,
+                                               const Cand &C, int si, int self_c, int cb) {
+    const double4 pi = load_const(A.spos, si);
+    uint64_t mask = __builtin_amdgcn_ballot_w64(pair_test<CUT>(A, H, C, si, pi));
+    const int off = self_c - cb;
+    if (off >= 0 && off < kWave) mask &= ~(1ull << off);
+    return mask;
+}
+
 /*
- * One wave per home cell. The home cell's neighbour cells, in the
- * reference's traversal order (z outer, y, x inner, each -n..n;
+ * Count pass. One wave per home cell. The home cell's neighbour cells, in
+ * the reference's traversal order (z outer, y, x inner, each -n..n;
  * neighbour_visit.hh:63-84), are concatenated into one candidate index
  * space: lanes take 64 consecutive candidates (coalesced loads, kept in
- * registers) and the wave then loops over the home atoms, testing every
- * candidate against one home atom per step. A ballot gives the accepted
- * lanes; for the fill, a lane's output slot is the home atom's cursor plus
- * the accepted lanes below it, so accepted pairs of one atom land in
- * consecutive slots in candidate order - the reference's output order - and
- * consecutive lanes write consecutive addresses.
- *
- * Home-atom state (count or write cursor) lives in lane t for home atom t;
- * home cells with more than 64 atoms are processed in groups of 64.
- *
- * The shift S = (raw_i - rel_j + x) / n (neighbour_visit.hh:111-114) is
- * computed without integer division: periodic directions give
- * S = w + img_i - img_j, with w = floor((c_i + x) / n) the wrap count of the
- * neighbour cell; non-periodic directions give S = img_i (trunc), since there
- * rel_j = c_i + x.
+ * registers) and the wave loops over the home atoms, testing all 64
+ * candidates against one home atom per step (the home atom's record comes
+ * by a scalar load). The ballot of atom t is written into lane t; its
+ * popcount is the count, and for the first kMaskChunks chunks it is stored
+ * as the (chunk, atom) accept mask so that the fill does not test again.
+ * Home atoms are taken in groups of 64.
  */
-template <int MODE, int CUT, bool HASHED>
-__global__ __launch_bounds__(kBlock) void k_pairs(PairArgs A) {
-    __shared__ int s_start[kWavesPerBlock][kWave];
-    __shared__ int s_begin[kWavesPerBlock][kWave];
-    __shared__ int s_xyz[kWavesPerBlock][kWave];
-    __shared__ int s_wrap[kWavesPerBlock][kWave];
-
+template <int CUT, bool HASHED>
+__device__ __forceinline__ void count_cell(const PairArgs &A, int h, SegLds &S) {
+    HomeCell H;
+    if (!load_home<HASHED>(A, h, H)) return;
     const int lane = lane_id();
-    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
-    const int h = blk * kWavesPerBlock + wib;
-    if (h >= A.home_cells) return;
-
-    const int m = A.cell_count[h];
-    if (m == 0) return;
-    const int hb = A.cell_first[h];
-
-    int sys = 0;
-    int c0, c1, c2;
-    if (!HASHED) {
-        sys = system_of(A.cell_base, A.nsys, h);
-    }
-    const SysGeom &g = A.geo[sys];
-    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-    if (!HASHED) {
-        const int lin = h - g.cell_base;
-        c0 = lin % n0;
-        c1 = (lin / n0) % n1;
-        c2 = lin / (n0 * n1);
-    } else {
-        const int64_t key = A.hkey[h];
-        c0 = static_cast<int>(key % n0);
-        c1 = static_cast<int>((key / n0) % n1);
-        c2 = static_cast<int>(key / (static_cast<int64_t>(n0) * n1));
-    }
-    const int nx = g.nxyz[0], ny = g.nxyz[1], nz = g.nxyz[2];
-    const int sx = 2 * nx + 1, sy = 2 * ny + 1;
-    const int nseg = g.nseg;
-    const int p0 = g.pbc[0], p1 = g.pbc[1], p2 = g.pbc[2];
-    const double cutoff_sq = g.cutoff_sq;
-    const double b00 = g.bin[0], b01 = g.bin[1], b02 = g.bin[2];
-    const double b10 = g.bin[3], b11 = g.bin[4], b12 = g.bin[5];
-    const double b20 = g.bin[6], b21 = g.bin[7], b22 = g.bin[8];
-    const int abase = g.atom_base;
-
-    int *const st = s_start[wib];
-    int *const sb = s_begin[wib];
-    int *const sxyz = s_xyz[wib];
-    int *const swr = s_wrap[wib];
-
-    for (int grp0 = 0; grp0 < m; grp0 += kWave) {
-        const int mg = min(kWave, m - grp0);
+    for (int grp0 = 0; grp0 < H.m; grp0 += kWave) {
+        const int mg = uniform(min(kWave, H.m - grp0));
         int acc = 0;
-        int64_t cursor = 0;
-        if (MODE != MODE_COUNT && lane < mg)
-            cursor = A.offsets[abase + A.sinfo[hb + grp0 + lane].w];
-
-        for (int sg = 0; sg < nseg; sg += kWave) {
-            const int nsg = min(kWave, nseg - sg);
-            int len = 0, begin = 0, pxyz = 0, pw = 0;
-            if (lane < nsg) {
-                const int seg = sg + lane;
-                const int x = seg % sx - nx;
-                const int rest = seg / sx;
-                const int y = rest % sy - ny;
-                const int z = rest / sy - nz;
-                int cj0 = c0 + x, cj1 = c1 + y, cj2 = c2 + z;
-                int w0 = 0, w1 = 0, w2 = 0;
-                bool ok = true;
-                if (p0) {
-                    w0 = floor_div(cj0, n0);
-                    cj0 -= w0 * n0;
-                } else {
-                    ok = ok && cj0 >= 0 && cj0 < n0;
-                }
-                if (p1) {
-                    w1 = floor_div(cj1, n1);
-                    cj1 -= w1 * n1;
-                } else {
-                    ok = ok && cj1 >= 0 && cj1 < n1;
-                }
-                if (p2) {
-                    w2 = floor_div(cj2, n2);
-                    cj2 -= w2 * n2;
-                } else {
-                    ok = ok && cj2 >= 0 && cj2 < n2;
-                }
-                if (ok) {
-                    if (HASHED)
-                        hashed_slice(A, n0, n1, cj0, cj1, cj2, begin, len);
-                    else
-                        dense_slice(A, g.cell_base, n0, n1, cj0, cj1, cj2, begin, len);
-                }
-                pxyz = pack3(x, y, z);
-                pw = pack3(w0, w1, w2);
-            }
-            const int incl = wave_incl_scan(len);
-            const int total = __shfl(incl, kWave - 1, kWave);
-            st[lane] = incl - len;
-            sb[lane] = begin;
-            sxyz[lane] = pxyz;
-            swr[lane] = pw;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-            for (int cb = 0; cb < total; cb += kWave) {
-                const int c = cb + lane;
-                const bool cv = c < total;
-                /* segment holding candidate c: the last one starting at or
-                   before c (empty segments never win: the covering one comes
-                   later in order) */
-                int p = 0;
-#pragma unroll
-                for (int step = 32; step > 0; step >>= 1)
-                    if (p + step < nsg && st[p + step] <= c) p += step;
-                const int sj = cv ? sb[p] + (c - st[p]) : hb;
-                int x, y, z, w0, w1, w2;
-                unpack3(sxyz[p], x, y, z);
-                unpack3(swr[p], w0, w1, w2);
-                /* off = (z b3 + y b2) + x b1 (neighbour_visit.hh:67,73-74,80-81) */
-                const double o0 = (z * b20 + y * b10) + x * b00;
-                const double o1 = (z * b21 + y * b11) + x * b01;
-                const double o2 = (z * b22 + y * b12) + x * b02;
-                const double4 pj = A.spos[sj];
-                const int4 ij = A.sinfo[sj];
-                const bool centre = (x == 0) && (y == 0) && (z == 0);
-                const int tj = CUT == CUT_PER_TYPE ? A.stype[sj] : 0;
-                const int ds0 = p0 ? w0 - ij.x : 0;
-                const int ds1 = p1 ? w1 - ij.y : 0;
-                const int ds2 = p2 ? w2 - ij.z : 0;
-
+        int k = 0;
+        for (int sg = 0; sg < H.nseg; sg += kWave) {
+            const int nsg = min(kWave, H.nseg - sg);
+            int self_start;
+            const int total = build_segments<HASHED>(A, H, sg, nsg, lane, S, &self_start);
+            const int self_base = self_start >= 0 ? self_start + grp0 : -(1 << 30);
+            for (int cb = 0; cb < total; cb += kWave, k++) {
+                Cand C;
+                load_candidate<CUT>(A, H, S, nsg, cb + lane, total, C);
+                uint32_t mlo = 0, mhi = 0;
                 for (int t = 0; t < mg; t++) {
-                    const int si = hb + grp0 + t;
-                    const double4 pi = A.spos[si];
-                    /* dr = (rel_j - rel_i) + off; r2 = (dx^2 + dy^2) + dz^2
-                       (neighbour_visit.hh:89-94) */
-                    const double d0 = (pj.x - pi.x) + o0;
-                    const double d1 = (pj.y - pi.y) + o1;
-                    const double d2 = (pj.z - pi.z) + o2;
-                    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-                    bool ok = cv && (r2 < cutoff_sq) && !(centre && sj == si);
-                    if (CUT == CUT_PER_ATOM) {
-                        const double cc = pi.w + pj.w;
-                        ok = ok && (r2 < cc * cc);
-                    } else if (CUT == CUT_PER_TYPE) {
-                        const int ti = A.stype[si];
-                        if (ti >= 0 && ti < A.ncut && tj >= 0 && tj < A.ncut)
-                            ok = ok && (r2 < A.pt_sq[ti * A.ncut + tj]);
-                    }
-                    const uint64_t mask = __ballot(ok);
-                    const int cnt = __popcll(mask);
-                    if (MODE == MODE_COUNT) {
-                        acc += (lane == t) ? cnt : 0;
-                    } else if (mask) {
-                        const int64_t base = __shfl(cursor, t, kWave);
-                        if (ok) {
-                            const int64_t pos = base + mbcnt64(mask);
-                            if (pos < A.capacity) {
-                                const int4 ii = A.sinfo[si];
-                                if (A.out_i) A.out_i[pos] = ii.w;
-                                if (A.out_j) A.out_j[pos] = ij.w;
-                                if (MODE == MODE_FILL64) {
-                                    if (A.out_D) {
-                                        double *D = static_cast<double *>(A.out_D) + 3 * pos;
-                                        D[0] = d0;
-                                        D[1] = d1;
-                                        D[2] = d2;
-                                    }
-                                    if (A.out_d) static_cast<double *>(A.out_d)[pos] = sqrt(r2);
-                                } else {
-                                    if (A.out_D) {
-                                        float *D = static_cast<float *>(A.out_D) + 3 * pos;
-                                        D[0] = static_cast<float>(d0);
-                                        D[1] = static_cast<float>(d1);
-                                        D[2] = static_cast<float>(d2);
-                                    }
-                                    if (A.out_d)
-                                        static_cast<float *>(A.out_d)[pos] =
-                                            static_cast<float>(sqrt(r2));
-                                }
-                                if (A.out_S) {
-                                    int32_t *S = A.out_S + 3 * pos;
-                                    S[0] = ii.x + ds0;
-                                    S[1] = ii.y + ds1;
-                                    S[2] = ii.z + ds2;
-                                }
-                            }
-                        }
-                        if (lane == t) cursor += cnt;
-                    }
+                    const int si = H.hb + grp0 + t;
+                    const uint64_t mask = chunk_mask<CUT>(A, H, C, si, self_base + t, cb);
+                    mlo = write_lane(mlo, static_cast<uint32_t>(mask), t);
+                    mhi = write_lane(mhi, static_cast<uint32_t>(mask >> 32), t);
                 }
+                acc += __popc(mlo) + __popc(mhi);
+                if (k < kMaskChunks && lane < mg)
+                    A.masks[static_cast<int64_t>(k) * A.mask_stride + H.hb + grp0 + lane] =
+                        (static_cast<uint64_t>(mhi) << 32) | mlo;
             }
             __builtin_amdgcn_wave_barrier();
         }
-        if (MODE == MODE_COUNT && lane < mg)
-            A.counts[abase + A.sinfo[hb + grp0 + lane].w] = acc;
+        if (lane < mg) A.counts[H.abase + A.sinfo[H.hb + grp0 + lane].w] = acc;
     }
+}
+
+/* Fill-pass LDS of one wave: segment table, home-atom table of the current
+   group, accept masks of the current chunk window ([chunk][atom]), the
+   window's per-atom pair prefix and each atom's running output slot.
+   Structure of arrays: consecutive lanes touch consecutive banks. */
+constexpr int kWindow = 8;
+struct FillLds {
+    SegLds seg;
+    double hx[kWave], hy[kWave], hz[kWave];
+    int hi[kWave], hm0[kWave], hm1[kWave], hm2[kWave];
+    uint64_t msk[kWindow][kWave];
+    int cpre[kWindow][kWave]; /* pairs of atom t in the window's chunks before kk */
+    int pre[kWave];
+    int64_t cur[kWave];
+};
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* Position of the r-th (from 0) set bit of m; r < popcount(m). */
+__device__ __forceinline__ int select_bit(uint64_t m, int r) {
+    const uint32_t lo = static_cast<uint32_t>(m), hi = static_cast<uint32_t>(m >> 32);
+    const int clo = __popc(lo);
+    const bool up = r >= clo;
+    uint32_t w = up ? hi : lo;
+    r -= up ? clo : 0;
+    int pos = up ? 32 : 0;
+#pragma unroll
+    for (int width = 16; width > 0; width >>= 1) {
+        const int c = __popc(w & ((1u << width) - 1u));
+        const bool u = r >= c;
+        r -= u ? c : 0;
+        w = u ? (w >> width) : w;
+        pos += u ? width : 0;
+    }
+    return pos;
+}
+
+/*
+ * Fill pass. Same traversal and chunking as the count; the accept mask of
+ * every (chunk, home atom) comes from the count pass (or is re-tested beyond
+ * kMaskChunks). Per window of up to kWindow chunks the masks go to LDS, a
+ * wave scan of the per-atom popcounts gives each atom's first pair in the
+ * window, and the window's pairs - ordered by atom, then chunk, then lane,
+ * i.e. by output slot - are written 64 per step, one per lane: the lane
+ * finds its atom (binary search of the prefixes), its chunk (running
+ * popcount) and its candidate (rank select in the mask), looks the candidate
+ * up again and writes i, j, D, d, S at the atom's running slot. One atom's
+ * pairs take consecutive slots in candidate order - the reference's output
+ * order - and consecutive lanes write consecutive slots, so every atom's
+ * output segment is written in one short coalesced burst.
+ */
+template <int MODE, int CUT, bool HASHED>
+__device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L) {
+    HomeCell H;
+    if (!load_home<HASHED>(A, h, H)) return;
+    const int lane = lane_id();
+    for (int grp0 = 0; grp0 < H.m; grp0 += kWave) {
+        const int mg = uniform(min(kWave, H.m - grp0));
+        if (lane < mg) {
+            const int si = H.hb + grp0 + lane;
+            const double4 pp = A.spos[si];
+            const int4 inf = A.sinfo[si];
+            L.hx[lane] = pp.x;
+            L.hy[lane] = pp.y;
+            L.hz[lane] = pp.z;
+            L.hi[lane] = inf.w;
+            L.hm0[lane] = inf.x;
+            L.hm1[lane] = inf.y;
+            L.hm2[lane] = inf.z;
+            L.cur[lane] = A.offsets[H.abase + inf.w];
+        }
+        int kbase = 0;
+        for (int sg = 0; sg < H.nseg; sg += kWave) {
+            const int nsg = min(kWave, H.nseg - sg);
+            int self_start;
+            const int total = build_segments<HASHED>(A, H, sg, nsg, lane, L.seg, &self_start);
+            const int nch = (total + kWave - 1) / kWave;
+            const int self_base = self_start >= 0 ? self_start + grp0 : -(1 << 30);
+            for (int kb0 = 0; kb0 < nch; kb0 += kWindow) {
+                const int nb = min(kWindow, nch - kb0);
+                /* 1. the window's accept masks, [chunk][atom] */
+                int cnt = 0;
+                for (int kk = 0; kk < nb; kk++) {
+                    const int k = kbase + kb0 + kk;
+                    uint64_t mk = 0;
+                    if (k < kMaskChunks) {
+                        if (lane < mg)
+                            mk = A.masks[static_cast<int64_t>(k) * A.mask_stride + H.hb + grp0 +
+                                         lane];
+                    } else {
+                        Cand C;
+                        const int cb = (kb0 + kk) * kWave;
+                        load_candidate<CUT>(A, H, L.seg, nsg, cb + lane, total, C);
+                        uint32_t mlo = 0, mhi = 0;
+                        for (int t = 0; t < mg; t++) {
+                            const uint64_t m =
+                                chunk_mask<CUT>(A, H, C, H.hb + grp0 + t, self_base + t, cb);
+                            mlo = write_lane(mlo, static_cast<uint32_t>(m), t);
+                            mhi = write_lane(mhi, static_cast<uint32_t>(m >> 32), t);
+                        }
+                        mk = (static_cast<uint64_t>(mhi) << 32) | mlo;
+                    }
+                    L.msk[kk][lane] = mk;
+                    L.cpre[kk][lane] = cnt;
+                    cnt += __popcll(mk);
+                }
+                /* 2. each atom's first pair in the window */
+                const int incl = wave_incl_scan(cnt);
+                const int tw = uniform(__shfl(incl, kWave - 1, kWave));
+                L.pre[lane] = incl - cnt;
+                wave_sync_lds();
+                /* 3. the window's pairs, 64 per step */
+                for (int qb = 0; qb < tw; qb += kWave) {
+                    const int q = qb + lane;
+                    if (q < tw) {
+                        const int t = last_at_or_below(L.pre, mg, q);
+                        const int rank = q - L.pre[t];
+                        /* chunk: the last whose running count is <= rank */
+                        int kk = 0;
+#pragma unroll
+                        for (int j = 1; j < kWindow; j++)
+                            kk += ((j < nb) & (L.cpre[j][t] <= rank)) ? 1 : 0;
+                        const uint64_t mk = L.msk[kk][t];
+                        const int r = rank - L.cpre[kk][t];
+                        const int c = (kb0 + kk) * kWave + select_bit(mk, r);
+                        const int p = find_segment(L.seg, nsg, c);
+                        const int sj = L.seg.sb[p] + (c - L.seg.st[p]);
+                        const int64_t pos = L.cur[t] + rank;
+                        const double4 pj = A.spos[sj];
+                        const int4 ij = A.sinfo[sj];
+                        if (pos < A.capacity) {
+                            /* D = (pj - pi) + off and r2 in the traversal's
+                               operation order: the bits the test saw */
+                            const double dx = (pj.x - L.hx[t]) + L.seg.so0[p];
+                            const double dy = (pj.y - L.hy[t]) + L.seg.so1[p];
+                            const double dz = (pj.z - L.hz[t]) + L.seg.so2[p];
+                            const double r2 = dx * dx + dy * dy + dz * dz;
+                            if (A.out_i) A.out_i[pos] = L.hi[t];
+                            if (A.out_j) A.out_j[pos] = ij.w;
+                            if (MODE == MODE_FILL64) {
+                                if (A.out_D) {
+                                    double *D = static_cast<double *>(A.out_D) + 3 * pos;
+                                    D[0] = dx;
+                                    D[1] = dy;
+                                    D[2] = dz;
+                                }
+                                if (A.out_d) static_cast<double *>(A.out_d)[pos] = sqrt(r2);
+                            } else {
+                                if (A.out_D) {
+                                    float *D = static_cast<float *>(A.out_D) + 3 * pos;
+                                    D[0] = static_cast<float>(dx);
+                                    D[1] = static_cast<float>(dy);
+                                    D[2] = static_cast<float>(dz);
+                                }
+                                if (A.out_d)
+                                    static_cast<float *>(A.out_d)[pos] = static_cast<float>(sqrt(r2));
+                            }
+                            if (A.out_S) {
+                                /* S = img_i + w - img_j (periodic), img_i
+                                   (non-periodic) */
+                                int w0, w1, w2;
+                                unpack3(L.seg.swr[p], w0, w1, w2);
+                                int32_t *S = A.out_S + 3 * pos;
+                                S[0] = L.hm0[t] + (H.p0 ? w0 - ij.x : 0);
+                                S[1] = L.hm1[t] + (H.p1 ? w1 - ij.y : 0);
+                                S[2] = L.hm2[t] + (H.p2 ? w2 - ij.z : 0);
+                            }
+                        }
+                    }
+                }
+                /* 4. advance the atoms' slots past this window */
+                wave_sync_lds();
+                if (lane < mg) L.cur[lane] += cnt;
+                wave_sync_lds();
+            }
+            kbase += nch;
+        }
+    }
+}
+
+template <int MODE, int CUT, bool HASHED>
+__global__ __launch_bounds__(kBlock) void k_pairs(PairArgs A) {
+    using Lds = typename std::conditional<MODE == MODE_COUNT, SegLds, FillLds>::type;
+    __shared__ Lds s_lds[kWavesPerBlock];
+    const int wib = uniform(threadIdx.x / kWave);
+    const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
+    const int h = blk * kWavesPerBlock + wib;
+    if (h >= A.home_cells) return;
+    if constexpr (MODE == MODE_COUNT)
+        count_cell<CUT, HASHED>(A, h, s_lds[wib]);
+    else
+        fill_cell<MODE, CUT, HASHED>(A, h, s_lds[wib]);
 }
 
 /* Device-side parameter set-up (one thread): keeps the whole build free of
