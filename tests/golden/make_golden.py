@@ -1,0 +1,201 @@
+"""Write the golden neighbour-list fixtures from the compiled reference core.
+
+Run in a checkout that has the reference sources next to it (oracle/Makefile
+builds them into oracle/_ref/libref_neighbours.so):
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Each case stores its inputs verbatim (positions, cell, the exact inv_cell
+bytes, pbc, cutoff specification, types) and the reference's outputs: the
+full arrays for small cases, SHA-256 digests of each array (int32 i, j, S;
+float64 d, D; C order) for larger ones, always with the pair count and the
+per-atom counts. Also the post-processing known answers (first_neighbours,
+get_jump_indicies, triplet_list) from the compiled reference.
+"""
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+
+FULL_LIMIT = 4000  # pairs below which full arrays are stored
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def cases():
+    """(name, positions, cell, pbc, cutoff, types) of every golden case."""
+    out = []
+    eye = np.eye(3)
+    one = np.array([[0.5, 0.5, 0.5]])
+    # tests/test_core.cc:32-62, tests/test_extension.py:102-119
+    for rc in (1.1, 1.5):
+        out.append((f"unit_cube_rc{rc}", one, eye, (True, True, True), rc, None))
+    for pbc in ((False, False, False), (True, False, False), (True, False, True)):
+        out.append((f"unit_cube_pbc{''.join('T' if p else 'F' for p in pbc)}", one, eye, pbc,
+                    1.1, None))
+    # tests/test_core.cc:75-122
+    g = np.array([[x, y, z] for x in range(2) for y in range(2) for z in range(2)], float) + 0.5
+    out.append(("sc_2x2x2", g, 2 * eye, (True, True, True), 1.1, None))
+    # tests/test_extension.py:211-231
+    out.append(("per_atom_radii", np.array([[0.0, 0, 0], [1.0, 0, 0], [2.5, 0, 0]]), 10 * eye,
+                (False,) * 3, np.array([0.4, 0.4, 1.2]), None))
+    out.append(("per_type_2x2", np.array([[0.0, 0, 0], [1.0, 0, 0], [2.0, 0, 0]]), 10 * eye,
+                (False,) * 3, np.array([[0.5, 1.5], [1.5, 0.5]]),
+                np.array([0, 1, 0], np.int32)))
+    rng = np.random.default_rng(11)
+    out.append(("per_type_out_of_range", rng.uniform(0, 8, (300, 3)), 8 * eye, (True,) * 3,
+                np.array([[1.0, 1.5, 1.2], [1.5, 2.0, 0.9], [1.2, 0.9, 1.7]]),
+                rng.integers(-1, 4, 300).astype(np.int32)))
+    # tests/test_extension.py:126-134
+    for pbc in (True, False, (True, False, True), (False, True, False)):
+        rng = np.random.default_rng(1234)
+        pb = (pbc,) * 3 if isinstance(pbc, bool) else pbc
+        out.append((f"cubic_random_{''.join('T' if p else 'F' for p in pb)}",
+                    rng.uniform(0, 5.0, (40, 3)), 5.0 * eye, pb, 1.7, None))
+    # tests/test_extension.py:137-148
+    cellt = np.array([[4.0, 0.0, 0.0], [1.3, 3.7, 0.0], [0.8, 0.6, 4.2]])
+    for pb in ((True,) * 3, (False,) * 3, (True, True, False)):
+        rng = np.random.default_rng(7)
+        out.append((f"triclinic_outside_{''.join('T' if p else 'F' for p in pb)}",
+                    rng.uniform(-0.3, 1.3, (30, 3)) @ cellt, cellt, pb, 1.6, None))
+    # tests/test_extension.py:163-174 (and the non-periodic shift of atoms
+    # more than a box length outside)
+    rng = np.random.default_rng(99)
+    pos = rng.uniform(0, 6.0, (25, 3))
+    pos[3] += 6.0 * eye[0]
+    pos[7] -= 2 * 6.0 * eye[1]
+    pos[11] += 3 * 6.0 * eye[2]
+    for pb in ((True,) * 3, (False,) * 3, (True, False, True)):
+        out.append((f"far_outside_{''.join('T' if p else 'F' for p in pb)}", pos, 6.0 * eye,
+                    pb, 1.5, None))
+    # tests/test_extension.py:151-160
+    for pb in ((True,) * 3, (False,) * 3, (True, False, True)):
+        rng = np.random.default_rng(3)
+        out.append((f"vacuum_box_{''.join('T' if p else 'F' for p in pb)}",
+                    rng.uniform(0, 6.0, (250, 3)), 120.0 * eye, pb, 1.5, None))
+    # boxes narrower than the cutoff (multi-image, |S| >= 2)
+    cells = np.array([[1.0, 0.0, 0.0], [0.2, 1.1, 0.0], [0.1, -0.3, 0.9]])
+    rng = np.random.default_rng(21)
+    tiny = rng.uniform(0, 1, (5, 3)) @ cells
+    for cut in (0.7, 1.5, 2.5, 3.3):
+        for pb in ((True,) * 3, (True, True, False)):
+            out.append((f"tiny_cell_rc{cut}_{''.join('T' if p else 'F' for p in pb)}", tiny,
+                        cells, pb, cut, None))
+    # pairs exactly at the cutoff (shell distances of a simple-cubic lattice)
+    gl = np.arange(6, dtype=float)
+    sc = np.stack(np.meshgrid(gl, gl, gl, indexing="ij"), -1).reshape(-1, 3)
+    for name, rc in (("1", 1.0), ("sqrt2", float(np.sqrt(2.0))), ("sqrt3", float(np.sqrt(3.0))),
+                     ("2", 2.0)):
+        out.append((f"sc_lattice_shell_{name}", sc, 6 * eye, (True,) * 3, rc, None))
+    # atoms exactly on cell faces
+    gf = np.arange(0, 8.0, 2.0)
+    faces = np.stack(np.meshgrid(gf, gf, gf, indexing="ij"), -1).reshape(-1, 3)
+    faces = np.concatenate([faces, faces + 1.0, faces - 8.0, faces + 8.0])
+    for pb in ((True,) * 3, (False,) * 3, (False, True, True)):
+        out.append((f"cell_faces_{''.join('T' if p else 'F' for p in pb)}", faces, 8 * eye, pb,
+                    2.0, None))
+    # fcc Al primitive cell, 12 neighbours at a/sqrt(2) (tests/test_neighbours.py:203-207)
+    a = 4.05
+    prim = 0.5 * a * np.array([[0.0, 1.0, 1.0], [1.0, 0.0, 1.0], [1.0, 1.0, 0.0]])
+    out.append(("fcc_al_primitive", np.zeros((1, 3)), prim, (True,) * 3, 3.1, None))
+    # more than 64 atoms in one cell
+    rng = np.random.default_rng(5)
+    out.append(("crowded_cells", rng.uniform(0, 4.0, (900, 3)), 4.0 * eye, (True,) * 3, 1.9,
+                None))
+    # the 1000-atom, 10 A, rc 2.5 A case and the GPU-suite scenarios of
+    # tests/test_neighbour_list_gpu.cc:95-127
+    rng = np.random.default_rng(1)
+    out.append(("uniform_1000_rc2.5", rng.uniform(0, 10, (1000, 3)), 10 * eye, (True,) * 3, 2.5,
+                None))
+    for n, L, rc, seed in ((2000, 12.0, 1.0, 1), (500, 3.5, 1.0, 2), (3000, 15.0, 2.0, 3)):
+        rng = np.random.default_rng(seed)
+        out.append((f"gpu_suite_{n}_{L}_{rc}", rng.uniform(0, L, (n, 3)), L * eye, (True,) * 3,
+                    rc, None))
+    rng = np.random.default_rng(11)
+    Lc = (4000 / 12.0) ** (1 / 3)
+    out.append(("sparse_vacuum_4000", rng.uniform(0, Lc, (4000, 3)), 40 * Lc * eye,
+                (True,) * 3, 1.0, None))
+    # random triclinic fuzz, mixed periodicity, atoms outside
+    for seed in range(12):
+        rng = np.random.default_rng(1000 + seed)
+        n = int(rng.integers(1, 400))
+        cell = np.diag(rng.uniform(2.0, 12.0, 3)) + rng.uniform(-1.5, 1.5, (3, 3)) * (seed % 3 > 0)
+        pb = tuple(bool(x) for x in rng.integers(0, 2, 3))
+        pos = rng.uniform(-0.4, 1.4, (n, 3)) @ cell
+        out.append((f"fuzz_{seed}", pos, cell, pb, float(rng.uniform(0.3, 4.0)), None))
+    return out
+
+
+def main():
+    if not oracle.ref_available():
+        sys.exit("compiled reference not found: run `make -C oracle ref` first")
+    index = {}
+    arrays = {}
+    for name, pos, cell, pbc, cutoff, types in cases():
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        cell = np.ascontiguousarray(cell, dtype=np.float64)
+        inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+        pbc = np.asarray(pbc, dtype=bool).reshape(3)
+        res = oracle.ref_neighbour_list("ijdDS", np.zeros(3), cell, inv, pbc, pos, cutoff,
+                                        types=types)
+        P = len(res["i"])
+        counts = np.bincount(res["i"], minlength=len(pos)).astype(np.int32)
+        arrays[f"{name}/positions"] = pos
+        arrays[f"{name}/cell"] = cell
+        arrays[f"{name}/inv_cell"] = inv
+        arrays[f"{name}/pbc"] = pbc
+        if np.ndim(cutoff) == 0:
+            arrays[f"{name}/cutoff"] = np.array(float(cutoff))
+        else:
+            arrays[f"{name}/cutoff"] = np.asarray(cutoff, dtype=np.float64)
+        if types is not None:
+            arrays[f"{name}/types"] = np.asarray(types, dtype=np.int32)
+        arrays[f"{name}/counts"] = counts
+        entry = {"npairs": int(P), "full": P <= FULL_LIMIT,
+                 "digest": {q: digest(res[q]) for q in "ijdDS"}}
+        if P <= FULL_LIMIT:
+            for q in "ijdDS":
+                arrays[f"{name}/{q}"] = res[q]
+        index[name] = entry
+        print(f"{name:32s} N={len(pos):5d} P={P:7d} {'full' if entry['full'] else 'digest'}")
+
+    # post-processing known answers
+    post = {}
+    fn_cases = [(5, [1, 1, 1, 1, 3, 3, 3]), (6, [0, 1, 2, 3, 4, 5]), (8, [0, 1, 2, 3, 5, 6]),
+                (8, [0, 1, 2, 3, 3, 5, 6]), (5, [])]
+    post["first_neighbours"] = [
+        {"n": n, "i": i, "seed": oracle.ref_first_neighbours(n, np.array(i, np.int32)).tolist()}
+        for n, i in fn_cases]
+    jump_cases = [[0, 0, 0, 1, 1, 1, 1, 1, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4], [0]]
+    post["get_jump_indicies"] = [
+        {"sorted": s, "out": oracle.ref_get_jump_indicies(np.array(s, np.int32)).tolist()}
+        for s in jump_cases]
+    first_i = [0, 2, 6, 10]
+    absd = [2.2] * 4 + [3.0] * 2 + [2.0] * 4
+    ij, ik = oracle.ref_triplet_list(np.array(first_i, np.int32))
+    ij2, ik2 = oracle.ref_triplet_list(np.array(first_i, np.int32), np.array(absd), 2.6)
+    post["triplet_list"] = [
+        {"first_i": first_i, "ij": ij.tolist(), "ik": ik.tolist()},
+        {"first_i": first_i, "absdist": absd, "cutoff": 2.6, "ij": ij2.tolist(),
+         "ik": ik2.tolist()}]
+
+    np.savez_compressed(os.path.join(HERE, "neighbour_list_golden.npz"), **arrays)
+    with open(os.path.join(HERE, "neighbour_list_golden.json"), "w") as f:
+        json.dump({"source": "reference CPU core (src/libneighbours, CellOrder::Linear) "
+                             "compiled by oracle/Makefile",
+                   "cases": index, "post": post}, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,243 @@
+"""Pinning the oracle (CPU restatement of the reference path).
+
+1. Bit-for-bit against the golden fixtures written from the compiled
+   reference core (tests/golden/make_golden.py): full arrays for small cases,
+   SHA-256 digests for larger ones.
+2. Bit-for-bit against the compiled reference itself, live, on random inputs
+   (only where oracle/_ref was built).
+3. Against what the mathematics fixes, independently of the reference: a
+   brute-force search over explicit periodic images (the algorithm of the
+   reference's tests/test_extension.py:50-92) for the pair multiset, the
+   contract D == r[j] - r[i] + S @ cell, d == |D|, symmetry, no self pairs.
+4. The reference's known answers (tests/test_core.cc, test_extension.py,
+   test_neighbours.py).
+5. Sensitivity: a planted one-ulp change in the oracle's inputs must break the
+   comparison (so the fixtures pin rounding, boundary decisions and order).
+"""
+
+import hashlib
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import ROOT, bits
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+with open(os.path.join(GOLDEN, "neighbour_list_golden.json")) as _f:
+    INDEX = json.load(_f)
+ARR = np.load(os.path.join(GOLDEN, "neighbour_list_golden.npz"))
+CASES = sorted(INDEX["cases"])
+
+
+def case_inputs(name):
+    g = lambda k: ARR[f"{name}/{k}"]  # noqa: E731
+    cutoff = g("cutoff")
+    cutoff = float(cutoff) if cutoff.ndim == 0 else cutoff
+    types = ARR[f"{name}/types"] if f"{name}/types" in ARR.files else None
+    return g("positions"), g("cell"), g("inv_cell"), g("pbc"), cutoff, types
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def check_against_golden(name, res):
+    entry = INDEX["cases"][name]
+    assert len(res["i"]) == entry["npairs"], name
+    for q in "ijdDS":
+        if entry["full"]:
+            ref = ARR[f"{name}/{q}"]
+            assert res[q].shape == ref.shape, (name, q)
+            assert np.array_equal(bits(res[q]), bits(ref)), (name, q)
+        assert digest(res[q]) == entry["digest"][q], (name, q)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_matches_golden(name):
+    pos, cell, inv, pbc, cutoff, types = case_inputs(name)
+    for backend in (-1, 0, 1):
+        res = oracle.neighbour_list("ijdDS", np.zeros(3), cell, inv, pbc, pos, cutoff,
+                                    types=types, backend=backend, with_counts=True)
+        check_against_golden(name, res)
+        assert np.array_equal(res["counts"], ARR[f"{name}/counts"])
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("seed", range(30))
+def test_oracle_matches_compiled_reference(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(0, 300))
+    cell = np.diag(rng.uniform(1.0, 10.0, 3)) + rng.uniform(-1, 1, (3, 3)) * (seed % 2)
+    pbc = rng.integers(0, 2, 3).astype(bool)
+    pos = rng.uniform(-0.5, 1.5, (n, 3)) @ cell
+    inv = np.linalg.inv(cell.T)
+    kind = seed % 3
+    if kind == 0:
+        cutoff = float(rng.uniform(0.3, 3.0))
+        types = None
+    elif kind == 1:
+        cutoff = rng.uniform(0.1, 1.5, n)
+        types = None
+    else:
+        cutoff = rng.uniform(0.5, 2.5, (3, 3))
+        types = rng.integers(-1, 4, n).astype(np.int32)
+    ref = oracle.ref_neighbour_list("ijdDS", np.zeros(3), cell, inv, pbc, pos, cutoff,
+                                    types=types)
+    for backend in (-1, 1):
+        got = oracle.neighbour_list("ijdDS", np.zeros(3), cell, inv, pbc, pos, cutoff,
+                                    types=types, backend=backend)
+        for q in "ijdDS":
+            assert np.array_equal(bits(got[q]), bits(ref[q])), (seed, q)
+
+
+def brute_force(cell, pos, pbc, cutoff):
+    """Explicit periodic images (algorithm of the reference's
+    tests/test_extension.py:50-92), returning (i, j, S) rows with S the shift
+    in the ORIGINAL (unwrapped) coordinates for periodic directions."""
+    cell = np.asarray(cell, float)
+    pbc = np.broadcast_to(pbc, (3,))
+    frac = pos @ np.linalg.inv(cell)
+    wrap = np.zeros_like(frac)
+    for k in range(3):
+        if pbc[k]:
+            wrap[:, k] = np.floor(frac[:, k])
+    w = (frac - wrap) @ cell
+    vol = abs(np.linalg.det(cell))
+    rng_ = []
+    for k in range(3):
+        if pbc[k]:
+            a, b = cell[(k + 1) % 3], cell[(k + 2) % 3]
+            rng_.append(int(np.ceil(cutoff / (vol / np.linalg.norm(np.cross(a, b))))))
+        else:
+            rng_.append(0)
+    rows = []
+    for s in itertools.product(*[range(-r, r + 1) for r in rng_]):
+        s = np.array(s)
+        d = w[None, :, :] + (s @ cell)[None, None, :] - w[:, None, :]
+        dist = np.linalg.norm(d, axis=2)
+        ii, jj = np.nonzero(dist < cutoff)
+        for i, j in zip(ii, jj):
+            if i == j and not s.any():
+                continue
+            S = s - wrap[j].astype(int) + wrap[i].astype(int)
+            rows.append((i, j, *S))
+    return sorted(rows)
+
+
+@pytest.mark.parametrize("pbc", [(True,) * 3, (False,) * 3, (True, False, True),
+                                 (False, True, False), (True, True, False)])
+@pytest.mark.parametrize("seed", [1234, 7, 2024])
+def test_oracle_vs_brute_force(pbc, seed):
+    rng = np.random.default_rng(seed)
+    cell = np.array([[4.0, 0.0, 0.0], [1.3, 3.7, 0.0], [0.8, 0.6, 4.2]]) * rng.uniform(0.8, 1.3)
+    pos = rng.uniform(-0.3, 1.3, (35, 3)) @ cell
+    cutoff = float(rng.uniform(0.8, 2.4))
+    inv = np.linalg.inv(cell.T)
+    res = oracle.neighbour_list("ijdDS", np.zeros(3), cell, inv, pbc, pos, cutoff)
+    pb = np.asarray(pbc, bool)
+    S = res["S"] * pb  # non-periodic shifts follow the reference's own rule
+    got = sorted(zip(res["i"].tolist(), res["j"].tolist(), *S.T.tolist()))
+    assert got == brute_force(cell, pos, pbc, cutoff)
+    # D == r[j] - r[i] + S @ cell (periodic shifts; non-periodic D is r_j - r_i
+    # except for the reference's far-outside quirk, absent at these positions)
+    np.testing.assert_allclose(res["D"], pos[res["j"]] - pos[res["i"]] + res["S"] @ cell,
+                               atol=1e-10)
+    np.testing.assert_allclose(res["d"], np.linalg.norm(res["D"], axis=1), atol=1e-12)
+    assert (np.bincount(res["i"], minlength=35) == np.bincount(res["j"], minlength=35)).all()
+    # i-sorted output
+    assert np.all(np.diff(res["i"]) >= 0)
+
+
+def test_order_within_atom_follows_traversal():
+    """Within one i, pairs come neighbour-cell by neighbour-cell (z, y, x from
+    -n to n) and, inside a cell, in ascending j (neighbour_visit.hh:63-85 with
+    the counting sort of cell_list.cc:77-80)."""
+    rng = np.random.default_rng(3)
+    cell = 9.0 * np.eye(3)
+    pos = rng.uniform(0, 9, (300, 3))
+    rc = 3.0
+    res = oracle.neighbour_list("ijS", np.zeros(3), cell, np.linalg.inv(cell.T), True, pos, rc)
+    n, _, bins, _ = oracle.grid_geometry(cell, rc)
+    raw = np.floor(pos @ np.linalg.inv(cell) * n).astype(int)
+    cj = raw % n
+    for i in range(0, 300, 37):
+        sel = res["i"] == i
+        j = res["j"][sel]
+        ci = raw[i] % n
+        # neighbour-cell offset of each pair, from the wrapped cells
+        off = (cj[j] - ci + 1) % n - 1
+        key = (off[:, 2] * 3 + off[:, 1]) * 3 + off[:, 0]
+        order = np.lexsort((j, key))
+        assert np.array_equal(order, np.arange(len(j)))
+
+
+def test_known_answers():
+    """tests/test_core.cc:32-178, tests/test_extension.py:102-119,211-231,
+    tests/test_neighbours.py:203-207 (fcc Al: 12 neighbours at 2.86378246)."""
+    def n_pairs(pos, cell, pbc, rc, types=None):
+        cell = np.asarray(cell, float)
+        return oracle.neighbour_list("ijd", np.zeros(3), cell, np.linalg.inv(cell.T), pbc,
+                                     np.asarray(pos, float), rc, types=types)
+    one = [[0.5, 0.5, 0.5]]
+    assert len(n_pairs(one, np.eye(3), True, 1.1)["i"]) == 6
+    assert len(n_pairs(one, np.eye(3), True, 1.5)["i"]) == 18
+    assert len(n_pairs(one, np.eye(3), False, 1.1)["i"]) == 0
+    assert len(n_pairs(one, np.eye(3), [True, False, False], 1.1)["i"]) == 2
+    assert len(n_pairs(one, np.eye(3), [True, False, True], 1.1)["i"]) == 4
+    a = 4.05
+    prim = 0.5 * a * np.array([[0.0, 1.0, 1.0], [1.0, 0.0, 1.0], [1.0, 1.0, 0.0]])
+    r = n_pairs([[0.0, 0.0, 0.0]], prim, True, 3.1)
+    assert len(r["i"]) == 12
+    np.testing.assert_allclose(r["d"], 2.86378246, atol=1e-8)
+    r = n_pairs([[0.0, 0, 0], [1.0, 0, 0], [2.0, 0, 0]], 10 * np.eye(3), False,
+                np.array([[0.5, 1.5], [1.5, 0.5]]), types=np.array([0, 1, 0], np.int32))
+    assert sorted(zip(r["i"].tolist(), r["j"].tolist())) == [(0, 1), (1, 0), (1, 2), (2, 1)]
+
+
+def test_zero_volume_error():
+    bad = np.array([[1.0, 0, 0], [2.0, 0, 0], [0, 0, 1.0]])
+    with pytest.raises(RuntimeError, match="Zero cell volume"):
+        oracle.neighbour_list("i", np.zeros(3), bad, np.zeros((3, 3)), True,
+                              np.array([[0.0, 0, 0]]), 1.0)
+
+
+@pytest.mark.parametrize("name,perturb", [
+    ("sc_lattice_shell_sqrt2", "cutoff"),      # boundary decision at r2 == rc^2
+    ("cell_faces_TTT", "inv_cell"),            # floor() of atoms on cell faces
+    ("triclinic_outside_TTT", "positions"),    # D rounding path
+])
+def test_planted_mistake_is_detected(name, perturb):
+    """One ulp in the oracle's input must change the result the fixture pins."""
+    pos, cell, inv, pbc, cutoff, types = case_inputs(name)
+    if perturb == "cutoff":
+        cutoff = float(np.nextafter(cutoff, 0.0))
+    elif perturb == "inv_cell":
+        inv = np.nextafter(inv, -np.inf)
+    else:
+        pos = pos.copy()
+        pos[0, 0] = np.nextafter(pos[0, 0], np.inf)
+    res = oracle.neighbour_list("ijdDS", np.zeros(3), cell, inv, pbc, pos, cutoff, types=types)
+    with pytest.raises(AssertionError):
+        check_against_golden(name, res)
+
+
+def test_post_processing_known_answers():
+    """first_neighbours / get_jump_indicies / triplet_list of the package
+    against the compiled reference's answers (tests/test_extension.py:277-337)."""
+    from matscipy_neighbours_amd import first_neighbours, get_jump_indicies, triplet_list
+    post = INDEX["post"]
+    for c in post["first_neighbours"]:
+        assert first_neighbours(c["n"], np.array(c["i"], np.int32)).tolist() == c["seed"]
+        assert oracle.first_neighbours(c["n"], np.array(c["i"], np.int32)).tolist() == c["seed"]
+    for c in post["get_jump_indicies"]:
+        assert get_jump_indicies(np.array(c["sorted"], np.int32)).tolist() == c["out"]
+    t0, t1 = post["triplet_list"]
+    ij, ik = triplet_list(np.array(t0["first_i"], np.int32))
+    assert ij.tolist() == t0["ij"] and ik.tolist() == t0["ik"]
+    ij, ik = triplet_list(np.array(t1["first_i"], np.int32), np.array(t1["absdist"]),
+                          t1["cutoff"])
+    assert ij.tolist() == t1["ij"] and ik.tolist() == t1["ik"]
