@@ -27,6 +27,52 @@ sys.path.insert(0, ROOT)
 METRIC = "pairs/s at 1M atoms rc=5Å (%HBM peak); batched systems/s vs per-system"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fill_fcc1M.json")
+KERNEL_SOURCE = os.path.join(ROOT, "matscipy-neighbours_amd", "csrc", "mnl_kernels.hip")
+FILL_KERNEL = "k_pairs<1, 0, false>"
+
+
+def _source_sha():
+    import hashlib
+    with open(KERNEL_SOURCE, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def _counter_means(path):
+    """Mean counter value (KiB) per kernel name of a rocprofv3 --pmc CSV."""
+    import csv
+    import re
+    agg = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            k = re.sub(r"\(.*", "", r["Kernel_Name"].replace("mnl::(anonymous namespace)::", "")
+                       .replace("void ", ""))
+            agg.setdefault(k, []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in agg.items()}
+
+
+def traffic_summary(fetch_csv, write_csv, cfetch_csv, cwrite_csv, commands):
+    """HBM bytes per fill launch from FETCH_SIZE / WRITE_SIZE runs, each scaled
+    by what the same counter reads for a plain 1 GiB copy (--calibrate-copy)."""
+    copy_bytes = float(1 << 30)
+    ff = copy_bytes / (_counter_means(cfetch_csv)["__amd_rocclr_copyBuffer"] * 1024)
+    wf = copy_bytes / (_counter_means(cwrite_csv)["__amd_rocclr_copyBuffer"] * 1024)
+    fetch = _counter_means(fetch_csv)[FILL_KERNEL] * 1024
+    write = _counter_means(write_csv)[FILL_KERNEL] * 1024
+    out = {
+        "kernel": FILL_KERNEL,
+        "workload": "fcc 64^3 cells (1048576 atoms), rc 5.0 A, ijdDS f64",
+        "fetch_size_bytes_raw": fetch, "write_size_bytes_raw": write,
+        "copy_calibration": {"fetch_factor": ff, "write_factor": wf,
+                             "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes},
+        "bytes_read_per_launch": fetch * ff, "bytes_written_per_launch": write * wf,
+        "bytes_per_launch": fetch * ff + write * wf,
+        "kernel_source_sha256": _source_sha(),
+        "commands": commands,
+    }
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+    print(json.dumps(out, indent=1))
 
 
 def _log(msg):
@@ -75,7 +121,24 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (to scale HBM "
                          "traffic counters) and exit")
+    ap.add_argument("--traffic-from", nargs=4,
+                    metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="write the fill kernel's HBM traffic summary from rocprofv3 --pmc "
+                         "counter CSVs and exit (no GPU needed)")
     args = ap.parse_args()
+    if args.traffic_from:
+        traffic_summary(*args.traffic_from, commands=[
+            "rocprofv3 --pmc FETCH_SIZE -f csv -d <dir> -o run -- python3 bench.py --steps 3 "
+            "--warmup 1 --no-cpu",
+            "rocprofv3 --pmc WRITE_SIZE -f csv -d <dir> -o run -- python3 bench.py --steps 3 "
+            "--warmup 1 --no-cpu",
+            "rocprofv3 --pmc FETCH_SIZE -f csv -d <dir> -o run -- python3 bench.py "
+            "--calibrate-copy --steps 3",
+            "rocprofv3 --pmc WRITE_SIZE -f csv -d <dir> -o run -- python3 bench.py "
+            "--calibrate-copy --steps 3",
+            "python3 bench.py --traffic-from <fetch csv> <write csv> <copy fetch csv> "
+            "<copy write csv>"])
+        return
     if args.calibrate_copy:
         import torch
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
@@ -170,13 +233,16 @@ def main():
     pipe_bytes = 24 * N + 52 * P
     fill_bytes = 52 * P + 56 * N
     achieved = fill_bytes / (fill_ms * 1e-3) / 1e9
+    # HBM traffic of the fill from the committed counter summary, valid only
+    # while the kernel source is the one it was measured on.
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
-        try:
-            with open(TRAFFIC_FILE) as f:
-                traffic = json.load(f).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+        with open(TRAFFIC_FILE) as f:
+            tf = json.load(f)
+        if tf.get("kernel_source_sha256") == _source_sha():
+            traffic = tf["bytes_per_launch"]
+        else:
+            _log("traffic summary is stale (kernel source changed): traffic = null")
 
     cpu = None
     if not args.no_cpu:
@@ -207,7 +273,7 @@ def main():
         "hbm_frac_pipeline": pipe_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_pairs<fill>",
+            "kernel": FILL_KERNEL,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
