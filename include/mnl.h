@@ -217,6 +217,66 @@ const int64_t *mnl_plan_offsets(const mnl_plan *plan);
 /* Backend chosen by the last count (MNL_BACKEND_DENSE or _HASHED). */
 int mnl_plan_backend(const mnl_plan *plan);
 
+/* ---------------------------------------------------------------- batches
+ *
+ * Many independent systems in one build: atoms concatenated in system
+ * order, each system with its own cell, pbc and scalar cutoff; every
+ * system's pairs are exactly those of a single-system build of it (indices
+ * i, j local to the system), concatenated in system order. There is no
+ * counterpart in the reference (whose API builds one system per call); the
+ * single-system semantics each system follows are those of
+ * src/libneighbours/neighbour_list.hh:98-104.
+ */
+
+/*
+ * Upload the geometry of a batch of nsys systems into the plan (host
+ * arrays): atom_offsets [nsys+1] (first atom of each system, ascending,
+ * atom_offsets[0] == 0), origins [nsys*3] (NULL = all zero), cells and
+ * inv_cells [nsys*9] (row-major, lattice vectors as rows; inv_cell as given,
+ * fractional = inv_cell . (r - origin)), pbcs [nsys*3] int32, cutoffs
+ * [nsys]. Grows the workspaces and waits for `stream`. The geometry stays in
+ * the plan for any number of batch counts until the plan is used for a
+ * single-system count.
+ */
+int mnl_plan_set_batch(mnl_plan *plan, void *stream, int32_t nsys,
+                       const int32_t *atom_offsets, const double *origins,
+                       const double *cells, const double *inv_cells,
+                       const int32_t *pbcs, const double *cutoffs);
+
+/* Count of a batch (positions: [device] float64 [N][3], all systems
+   concatenated): waits for the stream and returns the total pair count. */
+int mnl_plan_batch_count(mnl_plan *plan, void *stream, const double *positions,
+                         int64_t *npairs);
+
+/* Asynchronous batch count: never waits and never copies from the host;
+   the total goes to *npairs_device ([device] int64, may be NULL).
+   Capturable. Fill with mnl_plan_fill_async / mnl_plan_fill. */
+int mnl_plan_batch_count_async(mnl_plan *plan, void *stream,
+                               const double *positions, int64_t *npairs_device);
+
+/* [device] int64 [nsys+1]: the first pair of every system in the
+   concatenated output, and the total at index nsys (asynchronous). Works
+   for a single-system count too (nsys = 1). */
+int mnl_plan_system_offsets(mnl_plan *plan, void *stream,
+                            int64_t *system_pair_offsets);
+
+/* Systems in the plan's last count or batch. */
+int32_t mnl_plan_nsys(const mnl_plan *plan);
+
+/* A batch build (count, fill into caller buffers of `capacity` pairs,
+   per-system pair offsets, total pair count) captured once as a HIP graph,
+   replayed by mnl_graph_launch with one launch. Replays read the positions
+   buffer in place (update it between replays); the batch geometry and the
+   output buffers are fixed at capture. */
+typedef struct mnl_graph mnl_graph;
+int mnl_plan_batch_graph_create(mnl_plan *plan, const double *positions,
+                                int64_t capacity, int32_t *first, int32_t *secnd,
+                                void *distvec, void *absdist, int32_t *shift,
+                                int float32_output, int64_t *system_pair_offsets,
+                                int64_t *npairs_device, mnl_graph **graph);
+int mnl_graph_launch(mnl_graph *graph, void *stream);
+void mnl_graph_destroy(mnl_graph *graph);
+
 /* Diagnostic, no counterpart in the reference: y = sqrt(x) for n doubles
    (host buffers) computed by a device kernel, so that tests can pin that the
    device square root rounds exactly like the host's (d == sqrt(r2)). */

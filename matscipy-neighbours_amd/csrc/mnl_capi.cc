@@ -135,6 +135,8 @@ struct mnl_plan {
     int64_t npairs = 0;
     int32_t ncut = 0;
     hipStream_t stream = nullptr;
+    /* batched geometry uploaded by mnl_plan_set_batch and still in place */
+    bool batch_ready = false;
 };
 
 namespace {
@@ -165,9 +167,98 @@ PairArgs pair_args(const mnl_plan *p) {
 }
 
 /*
- * Enqueue bin -> scan -> scatter -> rank/gather -> count -> scan for one
- * system. Inputs are device pointers. Leaves the pair total in p->total
- * (device); does not synchronise.
+ * Grow the plan's workspaces for nat atoms, `cells` dense cells or hash
+ * slots and nsys systems. Growing frees old buffers, so pending work on the
+ * stream must finish first; a stream under graph capture cannot wait, and
+ * the caller runs one uncaptured build of the same size beforehand.
+ */
+int ensure_workspaces(mnl_plan *p, hipStream_t stream, int32_t nat, int64_t cells,
+                      bool hashed, int cut, int32_t ncutoffs, int32_t nsys) {
+    const size_t n = static_cast<size_t>(std::max(nat, 1));
+    const int64_t nparts =
+        std::max(scan_partial_size(cells), scan_partial_size(static_cast<int64_t>(n)));
+    struct Need {
+        DevBuf *b;
+        size_t bytes;
+    } needs[] = {
+        {&p->raw4, n * sizeof(int4)},
+        {&p->ticket, n * sizeof(int32_t)},
+        {&p->tmp, n * sizeof(int32_t)},
+        {&p->spos, n * sizeof(double4)},
+        {&p->sinfo, n * sizeof(int4)},
+        {&p->stype, cut == CUT_PER_TYPE ? n * sizeof(int32_t) : 0},
+        {&p->pt, cut == CUT_PER_TYPE
+                     ? sizeof(double) * std::max<int64_t>(1, int64_t(ncutoffs) * ncutoffs)
+                     : 0},
+        {&p->counts, n * sizeof(int32_t)},
+        {&p->masks, n * kMaskChunksHost * sizeof(uint64_t)},
+        {&p->offsets, (n + 1) * sizeof(int64_t)},
+        {&p->cell_count, static_cast<size_t>(std::max<int64_t>(cells, 1)) * sizeof(int32_t)},
+        {&p->cell_first, static_cast<size_t>(std::max<int64_t>(cells, 1)) * sizeof(int32_t)},
+        {&p->partial, static_cast<size_t>(nparts) * sizeof(int64_t)},
+        {&p->geo, sizeof(SysGeom) * static_cast<size_t>(nsys)},
+        {&p->atom_base, (static_cast<size_t>(nsys) + 1) * sizeof(int32_t)},
+        {&p->cell_base, (static_cast<size_t>(nsys) + 1) * sizeof(int32_t)},
+        {&p->total, sizeof(int64_t)},
+        {&p->hkey, hashed ? static_cast<size_t>(cells) * sizeof(int64_t) : 0},
+    };
+    bool grow = false;
+    for (const Need &nd : needs) grow = grow || nd.bytes > nd.b->cap;
+    if (!grow) return MNL_SUCCESS;
+    hipStreamCaptureStatus cap_status = hipStreamCaptureStatusNone;
+    MNL_HIP(hipStreamIsCapturing(stream, &cap_status));
+    if (cap_status != hipStreamCaptureStatusNone)
+        return fail("mnl workspace must grow during graph capture: run one "
+                    "uncaptured build of this size first");
+    MNL_HIP(hipStreamSynchronize(stream));
+    for (const Need &nd : needs) MNL_HIP(nd.b->ensure(nd.bytes));
+    return MNL_SUCCESS;
+}
+
+/*
+ * The launch sequence of a build whose geometry (geo, atom_base, cell_base)
+ * is already on the device: clear -> bin -> scan -> scatter -> rank/gather
+ * -> count -> scan. Leaves the pair total in p->total (device); does not
+ * synchronise and records no memset, so it can be captured in a graph.
+ */
+int enqueue_build(mnl_plan *p, hipStream_t stream, const double *positions,
+                  const double *per_atom, const int32_t *types) {
+    const bool hashed = p->backend == MNL_BACKEND_HASHED;
+    const int64_t cells = p->home_cells;
+    const int32_t nat = p->nat;
+    MNL_HIP(launch_fill_i32(p->cell_count.as<int32_t>(), cells, 0, stream));
+    if (hashed) {
+        MNL_HIP(launch_fill_i64(p->hkey.as<int64_t>(), cells, -1, stream));
+        MNL_HIP(launch_bin_hashed(positions, nat, p->geo.as<SysGeom>(), p->raw4.as<int4>(),
+                                  p->ticket.as<int32_t>(), p->hkey.as<int64_t>(), p->hmask,
+                                  p->cell_count.as<int32_t>(), stream));
+    } else {
+        MNL_HIP(launch_bin_dense(positions, nat, p->geo.as<SysGeom>(),
+                                 p->atom_base.as<int32_t>(), p->nsys, p->raw4.as<int4>(),
+                                 p->ticket.as<int32_t>(), p->cell_count.as<int32_t>(),
+                                 stream));
+    }
+    MNL_HIP(launch_scan_i32_to_i32(p->cell_count.as<int32_t>(), cells,
+                                   p->cell_first.as<int32_t>(), p->partial.as<int64_t>(),
+                                   nullptr, stream));
+    MNL_HIP(launch_scatter(p->raw4.as<int4>(), p->ticket.as<int32_t>(), nat,
+                           p->cell_first.as<int32_t>(), p->tmp.as<int32_t>(), stream));
+    MNL_HIP(launch_rank_gather(positions, p->raw4.as<int4>(), p->tmp.as<int32_t>(), nat,
+                               p->cell_first.as<int32_t>(), p->cell_count.as<int32_t>(),
+                               p->geo.as<SysGeom>(), p->atom_base.as<int32_t>(), p->nsys,
+                               per_atom, types, p->spos.as<double4>(), p->sinfo.as<int4>(),
+                               p->cut == CUT_PER_TYPE ? p->stype.as<int32_t>() : nullptr,
+                               stream));
+    PairArgs a = pair_args(p);
+    MNL_HIP(launch_pairs(0, p->cut, hashed, a, stream));
+    MNL_HIP(launch_scan_i32_to_i64(p->counts.as<int32_t>(), nat, p->offsets.as<int64_t>(),
+                                   p->partial.as<int64_t>(), p->total.as<int64_t>(), stream));
+    return MNL_SUCCESS;
+}
+
+/*
+ * Enqueue a whole count for one system (inputs are device pointers). Leaves
+ * the pair total in p->total (device); does not synchronise.
  */
 int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
                   const double cell[9], const double inv_cell[9], const int32_t pbc[3],
@@ -175,8 +266,8 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
                   const double *per_atom, const double *per_type_sq, int32_t ncutoffs,
                   const int32_t *types, int backend) {
     p->counted = false;
+    p->batch_ready = false;
     if (nat < 0) return fail("negative atom count");
-    if (!valid_quantities(0)) return fail("bad quantities");
     if (!(cutoff == cutoff)) return fail("cutoff is NaN");
 
     SysGeom g{};
@@ -206,7 +297,6 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
     p->ncut = p->cut == CUT_PER_TYPE ? ncutoffs : 0;
     p->stream = stream;
 
-    const size_t n = static_cast<size_t>(std::max(nat, 1));
     int64_t cells_or_slots = ncells;
     if (hashed) {
         int64_t cap = 1;
@@ -218,48 +308,9 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
     }
     p->home_cells = static_cast<int32_t>(cells_or_slots);
 
-    /* Workspaces only grow; growing frees the old buffer, so pending work on
-       the stream must finish first. A stream under graph capture cannot wait:
-       the caller runs one uncaptured build of the same size beforehand. */
-    const int64_t nparts =
-        std::max(scan_partial_size(cells_or_slots), scan_partial_size(static_cast<int64_t>(n)));
-    struct Need {
-        DevBuf *b;
-        size_t bytes;
-    } needs[] = {
-        {&p->raw4, n * sizeof(int4)},
-        {&p->ticket, n * sizeof(int32_t)},
-        {&p->tmp, n * sizeof(int32_t)},
-        {&p->spos, n * sizeof(double4)},
-        {&p->sinfo, n * sizeof(int4)},
-        {&p->stype, p->cut == CUT_PER_TYPE ? n * sizeof(int32_t) : 0},
-        {&p->pt, p->cut == CUT_PER_TYPE
-                     ? sizeof(double) * std::max<int64_t>(1, int64_t(ncutoffs) * ncutoffs)
-                     : 0},
-        {&p->counts, n * sizeof(int32_t)},
-        {&p->masks, n * kMaskChunksHost * sizeof(uint64_t)},
-        {&p->offsets, (n + 1) * sizeof(int64_t)},
-        {&p->cell_count, static_cast<size_t>(cells_or_slots) * sizeof(int32_t)},
-        {&p->cell_first, static_cast<size_t>(cells_or_slots) * sizeof(int32_t)},
-        {&p->partial, static_cast<size_t>(nparts) * sizeof(int64_t)},
-        {&p->geo, sizeof(SysGeom)},
-        {&p->atom_base, 2 * sizeof(int32_t)},
-        {&p->cell_base, 2 * sizeof(int32_t)},
-        {&p->total, sizeof(int64_t)},
-        {&p->hkey, hashed ? static_cast<size_t>(cells_or_slots) * sizeof(int64_t) : 0},
-    };
-    bool grow = false;
-    for (const Need &nd : needs) grow = grow || nd.bytes > nd.b->cap;
-    hipStreamCaptureStatus cap_status = hipStreamCaptureStatusNone;
-    MNL_HIP(hipStreamIsCapturing(stream, &cap_status));
-    const bool capturing = cap_status != hipStreamCaptureStatusNone;
-    if (grow) {
-        if (capturing)
-            return fail("mnl workspace must grow during graph capture: run one "
-                        "uncaptured build of this size first");
-        MNL_HIP(hipStreamSynchronize(stream));
-        for (const Need &nd : needs) MNL_HIP(nd.b->ensure(nd.bytes));
-    }
+    if (ensure_workspaces(p, stream, nat, cells_or_slots, hashed, p->cut, ncutoffs, 1) !=
+        MNL_SUCCESS)
+        return MNL_ERROR;
 
     /* parameters are written on the device by a one-thread kernel: no
        host-to-device copy, nothing read from host memory later */
@@ -267,40 +318,15 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
                               p->cell_base.as<int32_t>(), nat,
                               static_cast<int32_t>(hashed ? 0 : ncells), stream));
     if (p->cut == CUT_PER_TYPE && ncutoffs > 0) {
-        if (capturing) return fail("per-type cutoffs cannot be used under graph capture");
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        MNL_HIP(hipStreamIsCapturing(stream, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return fail("per-type cutoffs cannot be used under graph capture");
         const size_t b = sizeof(double) * size_t(ncutoffs) * size_t(ncutoffs);
         MNL_HIP(hipMemcpyAsync(p->pt.ptr, per_type_sq, b, hipMemcpyHostToDevice, stream));
         MNL_HIP(hipStreamSynchronize(stream));
     }
-
-    MNL_HIP(launch_fill_i32(p->cell_count.as<int32_t>(), cells_or_slots, 0, stream));
-    if (hashed) {
-        MNL_HIP(launch_fill_i64(p->hkey.as<int64_t>(), cells_or_slots, -1, stream));
-        MNL_HIP(launch_bin_hashed(positions, nat, p->geo.as<SysGeom>(), p->raw4.as<int4>(),
-                                  p->ticket.as<int32_t>(), p->hkey.as<int64_t>(), p->hmask,
-                                  p->cell_count.as<int32_t>(), stream));
-    } else {
-        MNL_HIP(launch_bin_dense(positions, nat, p->geo.as<SysGeom>(),
-                                 p->atom_base.as<int32_t>(), 1, p->raw4.as<int4>(),
-                                 p->ticket.as<int32_t>(), p->cell_count.as<int32_t>(),
-                                 stream));
-    }
-    MNL_HIP(launch_scan_i32_to_i32(p->cell_count.as<int32_t>(), cells_or_slots,
-                                   p->cell_first.as<int32_t>(), p->partial.as<int64_t>(),
-                                   nullptr, stream));
-    MNL_HIP(launch_scatter(p->raw4.as<int4>(), p->ticket.as<int32_t>(), nat,
-                           p->cell_first.as<int32_t>(), p->tmp.as<int32_t>(), stream));
-    MNL_HIP(launch_rank_gather(positions, p->raw4.as<int4>(), p->tmp.as<int32_t>(), nat,
-                               p->cell_first.as<int32_t>(), p->cell_count.as<int32_t>(),
-                               p->geo.as<SysGeom>(), p->atom_base.as<int32_t>(), 1, per_atom,
-                               types, p->spos.as<double4>(), p->sinfo.as<int4>(),
-                               p->cut == CUT_PER_TYPE ? p->stype.as<int32_t>() : nullptr,
-                               stream));
-    PairArgs a = pair_args(p);
-    MNL_HIP(launch_pairs(0, p->cut, hashed, a, stream));
-    MNL_HIP(launch_scan_i32_to_i64(p->counts.as<int32_t>(), nat, p->offsets.as<int64_t>(),
-                                   p->partial.as<int64_t>(), p->total.as<int64_t>(), stream));
-    return MNL_SUCCESS;
+    return enqueue_build(p, stream, positions, per_atom, types);
 }
 
 int read_total(mnl_plan *p, hipStream_t stream, int64_t *npairs) {
@@ -346,6 +372,12 @@ mnl_plan *shared_plan(int device) {
 }  // namespace
 
 /* ======================================================================== */
+
+struct mnl_graph {
+    int device = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
 
 struct mnl_result {
     int device = 0;
@@ -729,6 +761,185 @@ void mnl_result_free(mnl_result *r) {
     if (!r) return;
     free_result_buffers(r);
     delete r;
+}
+
+int mnl_plan_set_batch(mnl_plan *p, void *stream, int32_t nsys, const int32_t *atom_offsets,
+                       const double *origins, const double *cells, const double *inv_cells,
+                       const int32_t *pbcs, const double *cutoffs) {
+    if (!p) return fail("plan is NULL");
+    if (nsys < 1) return fail("a batch needs at least one system");
+    if (!atom_offsets || !cells || !inv_cells || !pbcs || !cutoffs)
+        return fail("batch geometry pointer is NULL");
+    if (atom_offsets[0] != 0) return fail("atom_offsets[0] must be 0");
+    DeviceScope ds(p->device);
+    if (!ds.ok) return fail("cannot switch to the plan's device");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    p->counted = false;
+    p->batch_ready = false;
+    std::vector<SysGeom> geo(static_cast<size_t>(nsys));
+    std::vector<int32_t> abase(static_cast<size_t>(nsys) + 1), cbase(static_cast<size_t>(nsys) + 1);
+    const double zero3[3] = {0.0, 0.0, 0.0};
+    int64_t ncells = 0;
+    for (int32_t s = 0; s < nsys; s++) {
+        const int32_t a0 = atom_offsets[s], a1 = atom_offsets[s + 1];
+        if (a1 < a0) return fail("atom_offsets must be non-decreasing");
+        const double rc = cutoffs[s];
+        if (!(rc == rc)) return fail("cutoff is NaN");
+        SysGeom &g = geo[static_cast<size_t>(s)];
+        if (!make_sys_geom(origins ? origins + 3 * int64_t(s) : zero3, cells + 9 * int64_t(s),
+                           inv_cells + 9 * int64_t(s), pbcs + 3 * int64_t(s), rc, a0, a1 - a0,
+                           static_cast<int32_t>(ncells), g))
+            return fail("Zero cell volume (system " + std::to_string(s) + ").");
+        for (int k = 0; k < 3; k++) {
+            if (g.nxyz[k] > 255)
+                return fail("cutoff spans more than 255 cells of the grid in one direction "
+                            "(system " + std::to_string(s) + ")");
+            if (g.nxyz[k] < 0) return fail("negative cutoff");
+        }
+        abase[static_cast<size_t>(s)] = a0;
+        cbase[static_cast<size_t>(s)] = static_cast<int32_t>(ncells);
+        ncells += static_cast<int64_t>(g.n[0]) * g.n[1] * g.n[2];
+        if (ncells > INT32_MAX / 2) return fail("batched cell grids too large");
+    }
+    const int32_t nat = atom_offsets[nsys];
+    abase[static_cast<size_t>(nsys)] = nat;
+    cbase[static_cast<size_t>(nsys)] = static_cast<int32_t>(ncells);
+    if (ensure_workspaces(p, st, nat, ncells, false, CUT_GLOBAL, 0, nsys) != MNL_SUCCESS)
+        return MNL_ERROR;
+    MNL_HIP(hipMemcpyAsync(p->geo.ptr, geo.data(), sizeof(SysGeom) * geo.size(),
+                           hipMemcpyHostToDevice, st));
+    MNL_HIP(hipMemcpyAsync(p->atom_base.ptr, abase.data(), sizeof(int32_t) * abase.size(),
+                           hipMemcpyHostToDevice, st));
+    MNL_HIP(hipMemcpyAsync(p->cell_base.ptr, cbase.data(), sizeof(int32_t) * cbase.size(),
+                           hipMemcpyHostToDevice, st));
+    MNL_HIP(hipStreamSynchronize(st));
+    p->nat = nat;
+    p->nsys = nsys;
+    p->backend = MNL_BACKEND_DENSE;
+    p->cut = CUT_GLOBAL;
+    p->ncut = 0;
+    p->hmask = 0;
+    p->home_cells = static_cast<int32_t>(ncells);
+    p->stream = st;
+    p->batch_ready = true;
+    return MNL_SUCCESS;
+}
+
+namespace {
+int batch_count_enqueue(mnl_plan *p, hipStream_t st, const double *positions,
+                        int64_t *npairs_device) {
+    if (!p->batch_ready)
+        return fail("no batch geometry in the plan: call mnl_plan_set_batch first (a "
+                    "single-system count on the plan replaces it)");
+    if (p->nat > 0 && !positions) return fail("positions is NULL");
+    p->counted = false;
+    if (enqueue_build(p, st, positions, nullptr, nullptr) != MNL_SUCCESS) return MNL_ERROR;
+    if (npairs_device)
+        MNL_HIP(hipMemcpyAsync(npairs_device, p->total.ptr, sizeof(int64_t),
+                               hipMemcpyDeviceToDevice, st));
+    p->counted = true;
+    p->npairs = -1;
+    return MNL_SUCCESS;
+}
+}  // namespace
+
+int mnl_plan_batch_count_async(mnl_plan *p, void *stream, const double *positions,
+                               int64_t *npairs_device) {
+    if (!p) return fail("plan is NULL");
+    DeviceScope ds(p->device);
+    if (!ds.ok) return fail("cannot switch to the plan's device");
+    return batch_count_enqueue(p, static_cast<hipStream_t>(stream), positions, npairs_device);
+}
+
+int mnl_plan_batch_count(mnl_plan *p, void *stream, const double *positions,
+                         int64_t *npairs) {
+    if (!p) return fail("plan is NULL");
+    DeviceScope ds(p->device);
+    if (!ds.ok) return fail("cannot switch to the plan's device");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (batch_count_enqueue(p, st, positions, nullptr) != MNL_SUCCESS) return MNL_ERROR;
+    return read_total(p, st, npairs);
+}
+
+int mnl_plan_system_offsets(mnl_plan *p, void *stream, int64_t *system_pair_offsets) {
+    if (!p) return fail("plan is NULL");
+    if (!p->counted) return fail("mnl_plan_system_offsets called without a count");
+    if (!system_pair_offsets) return fail("system_pair_offsets is NULL");
+    DeviceScope ds(p->device);
+    MNL_HIP(launch_system_offsets(p->offsets.as<int64_t>(), p->atom_base.as<int32_t>(),
+                                  p->nsys, system_pair_offsets,
+                                  static_cast<hipStream_t>(stream)));
+    return MNL_SUCCESS;
+}
+
+int32_t mnl_plan_nsys(const mnl_plan *p) { return p ? p->nsys : 0; }
+
+int mnl_plan_batch_graph_create(mnl_plan *p, const double *positions, int64_t capacity,
+                                int32_t *first, int32_t *secnd, void *distvec, void *absdist,
+                                int32_t *shift, int float32_output,
+                                int64_t *system_pair_offsets, int64_t *npairs_device,
+                                mnl_graph **out) {
+    if (!out) return fail("out is NULL");
+    *out = nullptr;
+    if (!p) return fail("plan is NULL");
+    if (!p->batch_ready) return fail("no batch geometry in the plan: call mnl_plan_set_batch");
+    if (capacity < 0) return fail("negative capacity");
+    DeviceScope ds(p->device);
+    if (!ds.ok) return fail("cannot switch to the plan's device");
+    hipStream_t cs = nullptr;
+    MNL_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        return fail(std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    }
+    int st = batch_count_enqueue(p, cs, positions, npairs_device);
+    if (st == MNL_SUCCESS) {
+        const int64_t saved = p->npairs;
+        p->npairs = capacity;
+        st = enqueue_fill(p, cs, float32_output ? 2 : 1, first, secnd, distvec, absdist, shift);
+        p->npairs = saved;
+    }
+    if (st == MNL_SUCCESS && system_pair_offsets) {
+        e = launch_system_offsets(p->offsets.as<int64_t>(), p->atom_base.as<int32_t>(),
+                                  p->nsys, system_pair_offsets, cs);
+        if (e != hipSuccess) st = fail(std::string("HIP error: ") + hipGetErrorString(e));
+    }
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(cs, &graph);
+    (void)hipStreamDestroy(cs);
+    if (st != MNL_SUCCESS) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return MNL_ERROR;
+    }
+    if (e != hipSuccess) return fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    auto *g = new mnl_graph();
+    g->device = p->device;
+    g->graph = graph;
+    g->exec = exec;
+    *out = g;
+    return MNL_SUCCESS;
+}
+
+int mnl_graph_launch(mnl_graph *g, void *stream) {
+    if (!g) return fail("graph is NULL");
+    DeviceScope ds(g->device);
+    MNL_HIP(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+    return MNL_SUCCESS;
+}
+
+void mnl_graph_destroy(mnl_graph *g) {
+    if (!g) return;
+    DeviceScope ds(g->device);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
 }
 
 int mnl_debug_sqrt(const double *x_host, double *y_host, int64_t n) {

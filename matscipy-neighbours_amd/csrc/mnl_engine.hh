@@ -124,6 +124,9 @@ hipError_t launch_set_params(SysGeom *geo, const SysGeom &g, int32_t *atom_base,
                              hipStream_t stream);
 hipError_t launch_sqrt_probe(const double *x, double *y, int64_t n,
                              hipStream_t stream);
+/* out[s] = offsets[atom_base[s]] for s = 0..nsys: each system's first pair. */
+hipError_t launch_system_offsets(const int64_t *offsets, const int32_t *atom_base,
+                                 int32_t nsys, int64_t *out, hipStream_t stream);
 
 }  // namespace mnl
 

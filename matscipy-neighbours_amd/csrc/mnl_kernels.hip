@@ -909,6 +909,13 @@ __global__ void k_set_params(SysGeom *__restrict__ geo, SysGeom g, int32_t *__re
     cell_base[1] = ncells;
 }
 
+__global__ void k_system_offsets(const int64_t *__restrict__ offsets,
+                                 const int32_t *__restrict__ atom_base, int32_t nsys,
+                                 int64_t *__restrict__ out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s <= nsys) out[s] = offsets[atom_base[s]];
+}
+
 __global__ void k_sqrt_probe(const double *__restrict__ x, double *__restrict__ y, int64_t n) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t < n) y[t] = sqrt(x[t]);
@@ -1045,6 +1052,14 @@ hipError_t launch_set_params(SysGeom *geo, const SysGeom &g, int32_t *atom_base,
                              int32_t *cell_base, int32_t nat, int32_t ncells,
                              hipStream_t stream) {
     k_set_params<<<1, kWave, 0, stream>>>(geo, g, atom_base, cell_base, nat, ncells);
+    return hipGetLastError();
+}
+
+hipError_t launch_system_offsets(const int64_t *offsets, const int32_t *atom_base,
+                                 int32_t nsys, int64_t *out, hipStream_t stream) {
+    if (nsys < 0) return hipSuccess;
+    k_system_offsets<<<grid_for(int64_t(nsys) + 1), kBlock, 0, stream>>>(offsets, atom_base,
+                                                                         nsys, out);
     return hipGetLastError();
 }
 

@@ -13,11 +13,13 @@ from .neighbours import (
     get_jump_indicies,
     mic,
     neighbour_list,
+    neighbour_list_batch,
     triplet_list,
 )
 
 __all__ = [
     "neighbour_list",
+    "neighbour_list_batch",
     "first_neighbours",
     "get_jump_indicies",
     "triplet_list",

@@ -56,6 +56,17 @@ SIGNATURES = [
     ("mnl_plan_counts", ctypes.c_int, [_vp, _vp, _vp]),
     ("mnl_plan_offsets", _vp, [_vp]),
     ("mnl_plan_backend", ctypes.c_int, [_vp]),
+    ("mnl_plan_set_batch", ctypes.c_int,
+     [_vp, _vp, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("mnl_plan_batch_count", ctypes.c_int, [_vp, _vp, _vp, _lp]),
+    ("mnl_plan_batch_count_async", ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    ("mnl_plan_system_offsets", ctypes.c_int, [_vp, _vp, _vp]),
+    ("mnl_plan_nsys", ctypes.c_int32, [_vp]),
+    ("mnl_plan_batch_graph_create", ctypes.c_int,
+     [_vp, _vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp,
+      ctypes.POINTER(_vp)]),
+    ("mnl_graph_launch", ctypes.c_int, [_vp, _vp]),
+    ("mnl_graph_destroy", None, [_vp]),
     ("mnl_debug_sqrt", ctypes.c_int, [_vp, _vp, ctypes.c_int64]),
 ]
 

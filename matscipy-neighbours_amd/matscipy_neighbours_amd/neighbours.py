@@ -25,6 +25,7 @@ from . import _lib
 
 __all__ = [
     "neighbour_list",
+    "neighbour_list_batch",
     "coordination",
     "first_neighbours",
     "get_jump_indicies",
@@ -376,6 +377,101 @@ def neighbour_list(quantities, atoms=None, cutoff=None, *, positions=None, cell=
     tensors = [out[q] for q in quantities]
     arrays = _export(tensors, want_dev, array_namespace)
     return arrays[0] if len(quantities) == 1 else tuple(arrays)
+
+
+def _batch_geometry(cells, pbc, cutoff, natoms, cell_origins):
+    """Per-system host arrays of a batch: atom offsets, origins, cells,
+    inv_cell = inv(cell.T) per system (as a single-system call computes it),
+    int32 pbc and float cutoffs."""
+    natoms = np.asarray(natoms, dtype=np.int64).reshape(-1)
+    nsys = natoms.shape[0]
+    if nsys < 1:
+        raise ValueError("a batch needs at least one system")
+    if np.any(natoms < 0):
+        raise ValueError("natoms must be non-negative")
+    offs = np.zeros(nsys + 1, dtype=np.int64)
+    np.cumsum(natoms, out=offs[1:])
+    if offs[-1] > np.iinfo(np.int32).max:
+        raise ValueError("too many atoms in the batch")
+    cells = np.ascontiguousarray(np.broadcast_to(np.asarray(cells, dtype=float), (nsys, 3, 3)))
+    invs = np.ascontiguousarray(np.stack([np.linalg.inv(c.T) for c in cells]))
+    pb = np.ascontiguousarray(np.broadcast_to(np.asarray(pbc, dtype=bool), (nsys, 3)),
+                              dtype=np.int32)
+    if isinstance(cutoff, (int, np.integer)) and not isinstance(cutoff, bool):
+        raise ValueError("cutoff must be a float (or one float per system)")
+    rc = np.ascontiguousarray(np.broadcast_to(np.asarray(cutoff, dtype=float), (nsys,)))
+    co = (np.zeros((nsys, 3)) if cell_origins is None else
+          np.ascontiguousarray(np.broadcast_to(np.asarray(cell_origins, dtype=float),
+                                               (nsys, 3))))
+    return nsys, offs.astype(np.int32), co, cells, invs, pb, rc
+
+
+def neighbour_list_batch(quantities, positions, cells, pbc, cutoff, natoms, *,
+                         cell_origins=None, device=None, array_namespace=None,
+                         out_dtype=None):
+    """Neighbour lists of many independent systems in one build.
+
+    No counterpart in the reference, whose API builds one system per call:
+    every system's pairs are exactly those ``neighbour_list`` gives for it
+    alone (i and j local to the system, same order, same bits), concatenated
+    in system order.
+
+    positions: (sum(natoms), 3) float64, systems concatenated (host or
+    device); cells: (nsys, 3, 3) lattice vectors as rows; pbc: (nsys, 3),
+    (3,) or a bool; cutoff: a float or one float per system; natoms: atoms
+    per system. Returns the requested quantities followed by
+    ``system_pair_offsets`` (nsys + 1,) int64: system s owns pairs
+    ``[offsets[s], offsets[s + 1])``.
+    """
+    flags = _quantity_flags(quantities)
+    f32 = _is_f32(out_dtype)
+    nsys, offs, co, ce, inv, pb, rc = _batch_geometry(cells, pbc, cutoff, natoms,
+                                                      cell_origins)
+    on_device = _is_on_device(positions)
+    want_dev, device_id = _resolve_device(device, on_device)
+    _lib.require_gpu()
+    torch = _torch()
+    if on_device:
+        pos_t = _device_positions(positions)
+        if device_id >= 0 and pos_t.device.index != device_id:
+            pos_t = pos_t.to(f"cuda:{device_id}")
+    else:
+        host_pos = np.ascontiguousarray(positions, dtype=float)
+        if host_pos.ndim != 2 or host_pos.shape[1] != 3:
+            raise ValueError("positions must have shape (n, 3)")
+        dev = device_id if device_id >= 0 else torch.cuda.current_device()
+        pos_t = torch.from_numpy(host_pos).to(f"cuda:{dev}")
+    if pos_t.shape[0] != int(offs[-1]):
+        raise ValueError("positions must hold sum(natoms) rows")
+    lib = _lib.load()
+    idx = pos_t.device.index if pos_t.device.index is not None else torch.cuda.current_device()
+    plan = _lib.plan_for(idx)
+    out = {}
+    kw = dict(device=pos_t.device)
+    with plan.lock, torch.cuda.device(idx):
+        sp = _lib.ctypes.c_void_p(torch.cuda.current_stream(idx).cuda_stream)
+        _lib.check(lib.mnl_plan_set_batch(plan.handle, sp, nsys, _lib.ptr(offs),
+                                          _lib.ptr(co), _lib.ptr(ce), _lib.ptr(inv),
+                                          _lib.ptr(pb), _lib.ptr(rc)))
+        npairs = _lib.ctypes.c_int64(0)
+        _lib.check(lib.mnl_plan_batch_count(plan.handle, sp, _lib.ptr(pos_t),
+                                            _lib.ctypes.byref(npairs)))
+        P = int(npairs.value)
+        fdt = torch.float32 if f32 else torch.float64
+        shapes = {"i": ((P,), torch.int32), "j": ((P,), torch.int32), "D": ((P, 3), fdt),
+                  "d": ((P,), fdt), "S": ((P, 3), torch.int32)}
+        for q in "ijDdS":
+            if flags & _lib.QUANTITY_BITS[q]:
+                out[q] = torch.empty(shapes[q][0], dtype=shapes[q][1], **kw)
+        if flags:
+            fill = lib.mnl_plan_fill_f32 if f32 else lib.mnl_plan_fill
+            _lib.check(fill(plan.handle, sp, _lib.ptr(out.get("i")), _lib.ptr(out.get("j")),
+                            _lib.ptr(out.get("D")), _lib.ptr(out.get("d")),
+                            _lib.ptr(out.get("S"))))
+        sys_off = torch.empty(nsys + 1, dtype=torch.int64, **kw)
+        _lib.check(lib.mnl_plan_system_offsets(plan.handle, sp, _lib.ptr(sys_off)))
+    tensors = [out[q] for q in quantities] + [sys_off]
+    return tuple(_export(tensors, want_dev, array_namespace))
 
 
 def coordination(atoms=None, cutoff=None, *, positions=None, cell=None, pbc=None,
