@@ -277,6 +277,29 @@ int mnl_plan_batch_graph_create(mnl_plan *plan, const double *positions,
 int mnl_graph_launch(mnl_graph *graph, void *stream);
 void mnl_graph_destroy(mnl_graph *graph);
 
+/* ---------------------------------------------------------------- gradients
+ *
+ * Backward of a pair list, no counterpart in the reference (which has no
+ * derivatives): with D_p = r[j] - r[i] + S_p . cell and d_p = |D_p| over an
+ * i-sorted list of npairs pairs (first/secnd/shift as written by the fill),
+ * given dL/dD (grad_distvec [npairs*3], may be NULL) and dL/dd
+ * (grad_absdist [npairs], may be NULL; then distvec and absdist are
+ * required), writes dL/dr into grad_positions [nat*3] float64 and dL/dcell
+ * (cell rows = lattice vectors, dL/dcell[a][b] = sum_p S_pa g_pb) into
+ * grad_cell [9] float64, where g_p = dL/dD_p + dL/dd_p D_p / d_p (the d
+ * term is dropped for d_p = 0). Both outputs are overwritten. distvec,
+ * absdist and the incoming gradients are float32 when float32 != 0 (the FP32
+ * output option), float64 otherwise. All pointers [device]; asynchronous.
+ * The cell gradient is deterministic; grad_positions sums the j-side
+ * contributions with float64 atomics (order-dependent in the last bits).
+ */
+int mnl_plan_pair_backward(mnl_plan *plan, void *stream, int64_t npairs, int32_t nat,
+                           const int32_t *first, const int32_t *secnd,
+                           const int32_t *shift, const void *distvec,
+                           const void *absdist, const void *grad_distvec,
+                           const void *grad_absdist, int float32,
+                           double *grad_positions, double *grad_cell);
+
 /* Diagnostic, no counterpart in the reference: y = sqrt(x) for n doubles
    (host buffers) computed by a device kernel, so that tests can pin that the
    device square root rounds exactly like the host's (d == sqrt(r2)). */

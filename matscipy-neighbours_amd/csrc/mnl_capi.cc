@@ -122,7 +122,7 @@ struct mnl_plan {
     /* workspaces */
     DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, counts,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
-        htypes, masks;
+        htypes, masks, cell_partial;
     int64_t *host_total = nullptr; /* pinned */
     /* state of the last count */
     bool counted = false;
@@ -575,7 +575,7 @@ void mnl_plan_destroy(mnl_plan *p) {
                       &p->stype,     &p->cell_count, &p->cell_first, &p->counts, &p->offsets,
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
                       &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
-                      &p->masks};
+                      &p->masks,     &p->cell_partial};
     for (DevBuf *b : bufs) b->release();
     if (p->host_total) (void)hipHostFree(p->host_total);
     delete p;
@@ -940,6 +940,39 @@ void mnl_graph_destroy(mnl_graph *g) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     delete g;
+}
+
+int mnl_plan_pair_backward(mnl_plan *p, void *stream, int64_t npairs, int32_t nat,
+                           const int32_t *first, const int32_t *secnd, const int32_t *shift,
+                           const void *distvec, const void *absdist,
+                           const void *grad_distvec, const void *grad_absdist,
+                           int float32, double *grad_positions, double *grad_cell) {
+    if (!p) return fail("plan is NULL");
+    if (npairs < 0 || nat < 0) return fail("negative size");
+    if (!grad_positions && nat > 0) return fail("grad_positions is NULL");
+    if (!grad_cell) return fail("grad_cell is NULL");
+    if (npairs > 0) {
+        if (!first || !secnd || !shift) return fail("i, j and S are required");
+        if (!grad_distvec && !grad_absdist) return fail("no incoming gradient");
+        if (grad_absdist && (!distvec || !absdist))
+            return fail("D and d are required for the gradient of d");
+    }
+    DeviceScope ds(p->device);
+    if (!ds.ok) return fail("cannot switch to the plan's device");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t need = sizeof(double) * 9 * static_cast<size_t>(backward_blocks(npairs));
+    if (need > p->cell_partial.cap) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        MNL_HIP(hipStreamIsCapturing(st, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return fail("mnl workspace must grow during graph capture");
+        MNL_HIP(hipStreamSynchronize(st));
+        MNL_HIP(p->cell_partial.ensure(need));
+    }
+    MNL_HIP(launch_pair_backward(npairs, nat, first, secnd, shift, distvec, absdist,
+                                 grad_distvec, grad_absdist, float32 != 0, grad_positions,
+                                 grad_cell, p->cell_partial.as<double>(), st));
+    return MNL_SUCCESS;
 }
 
 int mnl_debug_sqrt(const double *x_host, double *y_host, int64_t n) {

@@ -124,6 +124,16 @@ hipError_t launch_set_params(SysGeom *geo, const SysGeom &g, int32_t *atom_base,
                              hipStream_t stream);
 hipError_t launch_sqrt_probe(const double *x, double *y, int64_t n,
                              hipStream_t stream);
+/* Backward of D = r[j] - r[i] + S.cell, d = |D| (see mnl_kernels.hip):
+   overwrites grad_pos [nat*3] and grad_cell [9]; cell_partial needs
+   backward_blocks(npairs) * 9 doubles. D, d, gD, gd are float when f32. */
+constexpr int kBackwardMaxBlocks = 2048;
+int backward_blocks(int64_t npairs);
+hipError_t launch_pair_backward(int64_t npairs, int32_t nat, const int32_t *first,
+                                const int32_t *secnd, const int32_t *shift, const void *D,
+                                const void *d, const void *gD, const void *gd, bool f32,
+                                double *grad_pos, double *grad_cell, double *cell_partial,
+                                hipStream_t stream);
 /* out[s] = offsets[atom_base[s]] for s = 0..nsys: each system's first pair. */
 hipError_t launch_system_offsets(const int64_t *offsets, const int32_t *atom_base,
                                  int32_t nsys, int64_t *out, hipStream_t stream);

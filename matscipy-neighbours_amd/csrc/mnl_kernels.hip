@@ -21,6 +21,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
 
@@ -909,6 +910,118 @@ __global__ void k_set_params(SysGeom *__restrict__ geo, SysGeom g, int32_t *__re
     cell_base[1] = ncells;
 }
 
+/* ---------------------------------------------------------------- backward */
+
+/* Inclusive sum over the lanes of the same i-segment (pairs are i-sorted, so
+   equal keys are contiguous within the wave). */
+__device__ __forceinline__ double seg_incl_sum(double v, int key) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const double t = __shfl_up(v, o, kWave);
+        const int tk = __shfl_up(key, o, kWave);
+        if (lane >= o && tk == key) v += t;
+    }
+    return v;
+}
+
+/*
+ * Backward of D_p = r[j] - r[i] + S_p . cell and d_p = |D_p| over an
+ * i-sorted pair list. Per pair g = dL/dD + (dL/dd) D / d (the d term is
+ * dropped for coincident atoms, d = 0). dL/dr[j] += g by f64 atomics;
+ * dL/dr[i] -= g summed first over each i-segment inside the wave (one
+ * atomic per segment per 64 pairs); dL/dcell[a][b] = sum_p S_pa g_pb kept
+ * per thread over a grid-stride loop, reduced per block into
+ * cell_partial[block][9] (fixed grid, fixed order: deterministic).
+ */
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pair_backward(
+    int64_t npairs, const int32_t *__restrict__ first, const int32_t *__restrict__ secnd,
+    const int32_t *__restrict__ shift, const T *__restrict__ D, const T *__restrict__ d,
+    const T *__restrict__ gD, const T *__restrict__ gd, double *__restrict__ grad_pos,
+    double *__restrict__ cell_partial) {
+    __shared__ double s_red[kWavesPerBlock][9];
+    double c[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) c[k] = 0.0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    /* every lane runs the same trip count so that the wave shuffles stay
+       converged; lanes past the end carry key -1 and zeros */
+    const int64_t base0 = static_cast<int64_t>(blockIdx.x) * kBlock + (threadIdx.x & ~(kWave - 1));
+    for (int64_t wb = base0; wb < npairs; wb += stride) {
+        const int64_t p = wb + lane_id();
+        const bool valid = p < npairs;
+        int i = -1, j = -1;
+        double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        if (valid) {
+            i = first[p];
+            j = secnd[p];
+            if (gD) {
+                g0 = static_cast<double>(gD[3 * p + 0]);
+                g1 = static_cast<double>(gD[3 * p + 1]);
+                g2 = static_cast<double>(gD[3 * p + 2]);
+            }
+            if (gd) {
+                const double dd = static_cast<double>(d[p]);
+                if (dd > 0.0) {
+                    const double f = static_cast<double>(gd[p]) / dd;
+                    g0 += f * static_cast<double>(D[3 * p + 0]);
+                    g1 += f * static_cast<double>(D[3 * p + 1]);
+                    g2 += f * static_cast<double>(D[3 * p + 2]);
+                }
+            }
+            s0 = static_cast<double>(shift[3 * p + 0]);
+            s1 = static_cast<double>(shift[3 * p + 1]);
+            s2 = static_cast<double>(shift[3 * p + 2]);
+            unsafeAtomicAdd(&grad_pos[3 * static_cast<int64_t>(j) + 0], g0);
+            unsafeAtomicAdd(&grad_pos[3 * static_cast<int64_t>(j) + 1], g1);
+            unsafeAtomicAdd(&grad_pos[3 * static_cast<int64_t>(j) + 2], g2);
+        }
+        c[0] += s0 * g0; c[1] += s0 * g1; c[2] += s0 * g2;
+        c[3] += s1 * g0; c[4] += s1 * g1; c[5] += s1 * g2;
+        c[6] += s2 * g0; c[7] += s2 * g1; c[8] += s2 * g2;
+        const double a0 = seg_incl_sum(g0, i);
+        const double a1 = seg_incl_sum(g1, i);
+        const double a2 = seg_incl_sum(g2, i);
+        const int next = __shfl_down(i, 1, kWave);
+        const bool tail = lane_id() == kWave - 1 || next != i;
+        if (valid && tail) {
+            unsafeAtomicAdd(&grad_pos[3 * static_cast<int64_t>(i) + 0], -a0);
+            unsafeAtomicAdd(&grad_pos[3 * static_cast<int64_t>(i) + 1], -a1);
+            unsafeAtomicAdd(&grad_pos[3 * static_cast<int64_t>(i) + 2], -a2);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) c[k] = wave_sum(c[k]);
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) s_red[w][k] = c[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        double t = 0.0;
+        for (int q = 0; q < kWavesPerBlock; q++) t += s_red[q][threadIdx.x];
+        cell_partial[static_cast<int64_t>(blockIdx.x) * 9 + threadIdx.x] = t;
+    }
+}
+
+/* Sum of nb per-block cell partials in a fixed order (one wave). */
+__global__ void k_cell_reduce(const double *__restrict__ partial, int nb,
+                              double *__restrict__ grad_cell) {
+    const int k = threadIdx.x;
+    if (k >= 9) return;
+    double t = 0.0;
+    for (int b = 0; b < nb; b++) t += partial[static_cast<int64_t>(b) * 9 + k];
+    grad_cell[k] = t;
+}
+
+__global__ void k_fill_f64(double *p, int64_t n, double v) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
 __global__ void k_system_offsets(const int64_t *__restrict__ offsets,
                                  const int32_t *__restrict__ atom_base, int32_t nsys,
                                  int64_t *__restrict__ out) {
@@ -1060,6 +1173,38 @@ hipError_t launch_system_offsets(const int64_t *offsets, const int32_t *atom_bas
     if (nsys < 0) return hipSuccess;
     k_system_offsets<<<grid_for(int64_t(nsys) + 1), kBlock, 0, stream>>>(offsets, atom_base,
                                                                          nsys, out);
+    return hipGetLastError();
+}
+
+int backward_blocks(int64_t npairs) {
+    const int64_t want = (npairs + kBlock - 1) / kBlock;
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, kBackwardMaxBlocks)));
+}
+
+hipError_t launch_pair_backward(int64_t npairs, int32_t nat, const int32_t *first,
+                                const int32_t *secnd, const int32_t *shift, const void *D,
+                                const void *d, const void *gD, const void *gd, bool f32,
+                                double *grad_pos, double *grad_cell, double *cell_partial,
+                                hipStream_t stream) {
+    if (nat > 0)
+        k_fill_f64<<<grid_for(int64_t(nat) * 3), kBlock, 0, stream>>>(grad_pos,
+                                                                     int64_t(nat) * 3, 0.0);
+    const int nb = backward_blocks(npairs);
+    if (npairs > 0) {
+        if (f32)
+            k_pair_backward<float><<<nb, kBlock, 0, stream>>>(
+                npairs, first, secnd, shift, static_cast<const float *>(D),
+                static_cast<const float *>(d), static_cast<const float *>(gD),
+                static_cast<const float *>(gd), grad_pos, cell_partial);
+        else
+            k_pair_backward<double><<<nb, kBlock, 0, stream>>>(
+                npairs, first, secnd, shift, static_cast<const double *>(D),
+                static_cast<const double *>(d), static_cast<const double *>(gD),
+                static_cast<const double *>(gd), grad_pos, cell_partial);
+        k_cell_reduce<<<1, kWave, 0, stream>>>(cell_partial, nb, grad_cell);
+    } else {
+        k_fill_f64<<<1, kWave, 0, stream>>>(grad_cell, 9, 0.0);
+    }
     return hipGetLastError();
 }
 

@@ -6,6 +6,7 @@ HIP kernels (libmnl.so, C ABI in include/mnl.h). Pairs, their order and the
 bits of D and d are identical to the reference CPU path.
 """
 
+from .autograd import neighbour_list_autograd
 from .neighbours import (
     DLPackTensor,
     coordination,
@@ -20,6 +21,7 @@ from .neighbours import (
 __all__ = [
     "neighbour_list",
     "neighbour_list_batch",
+    "neighbour_list_autograd",
     "first_neighbours",
     "get_jump_indicies",
     "triplet_list",

@@ -67,6 +67,9 @@ SIGNATURES = [
       ctypes.POINTER(_vp)]),
     ("mnl_graph_launch", ctypes.c_int, [_vp, _vp]),
     ("mnl_graph_destroy", None, [_vp]),
+    ("mnl_plan_pair_backward", ctypes.c_int,
+     [_vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+      ctypes.c_int, _vp, _vp]),
     ("mnl_debug_sqrt", ctypes.c_int, [_vp, _vp, ctypes.c_int64]),
 ]
 
