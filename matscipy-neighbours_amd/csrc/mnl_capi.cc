@@ -122,7 +122,7 @@ struct mnl_plan {
     /* workspaces */
     DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, counts,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
-        htypes, masks, cell_partial;
+        htypes, cell_partial, cell_key;
     int64_t *host_total = nullptr; /* pinned */
     /* state of the last count */
     bool counted = false;
@@ -161,8 +161,6 @@ PairArgs pair_args(const mnl_plan *p) {
     a.counts = p->counts.as<int32_t>();
     a.offsets = p->offsets.as<int64_t>();
     a.capacity = p->npairs;
-    a.masks = p->masks.as<uint64_t>();
-    a.mask_stride = p->nat;
     return a;
 }
 
@@ -191,10 +189,10 @@ int ensure_workspaces(mnl_plan *p, hipStream_t stream, int32_t nat, int64_t cell
                      ? sizeof(double) * std::max<int64_t>(1, int64_t(ncutoffs) * ncutoffs)
                      : 0},
         {&p->counts, n * sizeof(int32_t)},
-        {&p->masks, n * kMaskChunksHost * sizeof(uint64_t)},
         {&p->offsets, (n + 1) * sizeof(int64_t)},
         {&p->cell_count, static_cast<size_t>(std::max<int64_t>(cells, 1)) * sizeof(int32_t)},
         {&p->cell_first, static_cast<size_t>(std::max<int64_t>(cells, 1)) * sizeof(int32_t)},
+        {&p->cell_key, static_cast<size_t>(std::max<int64_t>(cells, 1)) * sizeof(int32_t)},
         {&p->partial, static_cast<size_t>(nparts) * sizeof(int64_t)},
         {&p->geo, sizeof(SysGeom) * static_cast<size_t>(nsys)},
         {&p->atom_base, (static_cast<size_t>(nsys) + 1) * sizeof(int32_t)},
@@ -227,20 +225,25 @@ int enqueue_build(mnl_plan *p, hipStream_t stream, const double *positions,
     const int64_t cells = p->home_cells;
     const int32_t nat = p->nat;
     MNL_HIP(launch_fill_i32(p->cell_count.as<int32_t>(), cells, 0, stream));
+    MNL_HIP(launch_fill_i32(p->cell_key.as<int32_t>(), cells, INT32_MAX, stream));
     if (hashed) {
         MNL_HIP(launch_fill_i64(p->hkey.as<int64_t>(), cells, -1, stream));
         MNL_HIP(launch_bin_hashed(positions, nat, p->geo.as<SysGeom>(), p->raw4.as<int4>(),
                                   p->ticket.as<int32_t>(), p->hkey.as<int64_t>(), p->hmask,
-                                  p->cell_count.as<int32_t>(), stream));
+                                  p->cell_count.as<int32_t>(), p->cell_key.as<int32_t>(),
+                                  stream));
     } else {
         MNL_HIP(launch_bin_dense(positions, nat, p->geo.as<SysGeom>(),
                                  p->atom_base.as<int32_t>(), p->nsys, p->raw4.as<int4>(),
                                  p->ticket.as<int32_t>(), p->cell_count.as<int32_t>(),
-                                 stream));
+                                 p->cell_key.as<int32_t>(), stream));
     }
-    MNL_HIP(launch_scan_i32_to_i32(p->cell_count.as<int32_t>(), cells,
-                                   p->cell_first.as<int32_t>(), p->partial.as<int64_t>(),
-                                   nullptr, stream));
+    /* slices laid out by each cell's smallest atom (tmp and counts hold the
+       weights and their scan until the scatter and the count overwrite them) */
+    MNL_HIP(launch_cell_order(p->raw4.as<int4>(), nat, p->cell_key.as<int32_t>(),
+                              p->cell_count.as<int32_t>(), cells, p->tmp.as<int32_t>(),
+                              p->counts.as<int32_t>(), p->partial.as<int64_t>(),
+                              p->cell_first.as<int32_t>(), stream));
     MNL_HIP(launch_scatter(p->raw4.as<int4>(), p->ticket.as<int32_t>(), nat,
                            p->cell_first.as<int32_t>(), p->tmp.as<int32_t>(), stream));
     MNL_HIP(launch_rank_gather(positions, p->raw4.as<int4>(), p->tmp.as<int32_t>(), nat,
@@ -575,7 +578,7 @@ void mnl_plan_destroy(mnl_plan *p) {
                       &p->stype,     &p->cell_count, &p->cell_first, &p->counts, &p->offsets,
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
                       &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
-                      &p->masks,     &p->cell_partial};
+                      &p->cell_partial, &p->cell_key};
     for (DevBuf *b : bufs) b->release();
     if (p->host_total) (void)hipHostFree(p->host_total);
     delete p;

@@ -6,9 +6,12 @@
  *   ticket int32  [N]  arrival slot inside the cell (from the histogram atomic)
  *   tmp    int32  [N]  atoms scattered into cell slices, arbitrary order
  *   spos   double4[N]  cell-sorted: bin-relative position xyz + per-atom radius
- *   sinfo  int4   [N]  cell-sorted: image counts xyz + atom index (system-local)
+ *   sinfo  int4   [N]  cell-sorted: per direction the raw cell (periodic) or
+ *                      the image count (non-periodic), + atom index (system-local)
  *   stype  int32  [N]  cell-sorted atom type (per-type cutoffs only)
- *   cell_count / cell_first int32 [C] dense CSR (or hash slots when hashed)
+ *   cell_count / cell_first int32 [C] dense CSR (or hash slots when hashed);
+ *                      slices laid out by each cell's smallest atom index
+ *   cell_key int32 [C] smallest atom index of each cell
  *   counts int32  [N]  neighbours per atom, original order
  *   offsets int64 [N+1] exclusive scan of counts (CSR row pointers)
  * A "system" is one periodic cell with its atoms; the plain path is a batch
@@ -30,6 +33,7 @@ struct SysGeom {
     double origin[3];
     double bin[9];      /* bin[3*k + m] = cell[3*k + m] / n[k] (cell edge vectors) */
     double cutoff_sq;
+    double rn[3];       /* 1.0 / n[k] (exact division of shift numerators) */
     int32_t n[3];       /* cells per lattice direction */
     int32_t nxyz[3];    /* neighbour cells scanned each way */
     int32_t pbc[3];
@@ -75,8 +79,6 @@ struct PairArgs {
     int32_t *counts;           /* count: [N] by global atom index */
     const int64_t *offsets;    /* fill: [N+1] */
     int64_t capacity;          /* fill: pairs beyond this are not written */
-    uint64_t *masks;           /* [kMaskChunks][mask_stride] accept masks */
-    int64_t mask_stride;       /* total atoms */
     int32_t *out_i;
     int32_t *out_j;
     void *out_D;
@@ -89,11 +91,19 @@ hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t stream)
 hipError_t launch_fill_i64(int64_t *p, int64_t n, int64_t v, hipStream_t stream);
 hipError_t launch_bin_dense(const double *r, int32_t nat, const SysGeom *geo,
                             const int32_t *atom_base, int32_t nsys, int4 *raw4,
-                            int32_t *ticket, int32_t *cell_count,
+                            int32_t *ticket, int32_t *cell_count, int32_t *cell_key,
                             hipStream_t stream);
 hipError_t launch_bin_hashed(const double *r, int32_t nat, const SysGeom *geo,
                              int4 *raw4, int32_t *ticket, int64_t *hkey,
-                             int64_t hmask, int32_t *hcount, hipStream_t stream);
+                             int64_t hmask, int32_t *hcount, int32_t *cell_key,
+                             hipStream_t stream);
+/* Cell slices ordered by each cell's smallest atom index (cell_key, from the
+   binning's atomicMin): cell_first for every cell. w and wscan need nat
+   entries each, partial scan_partial_size(nat). */
+hipError_t launch_cell_order(const int4 *raw4, int32_t nat, const int32_t *cell_key,
+                             const int32_t *cell_count, int64_t ncells, int32_t *w,
+                             int32_t *wscan, int64_t *partial, int32_t *cell_first,
+                             hipStream_t stream);
 hipError_t launch_scatter(const int4 *raw4, const int32_t *ticket, int32_t nat,
                           const int32_t *cell_first, int32_t *tmp,
                           hipStream_t stream);
@@ -114,9 +124,8 @@ hipError_t launch_scan_i32_to_i32(const int32_t *in, int64_t n, int32_t *out,
 hipError_t launch_scan_i32_to_i64(const int32_t *in, int64_t n, int64_t *out,
                                   int64_t *partial, int64_t *total,
                                   hipStream_t stream);
-/* Accept-mask chunks kept per atom by the count pass (see mnl_kernels.hip). */
-constexpr int kMaskChunksHost = 16;
-/* Pair kernels. mode 0 = count, 1 = fill f64 outputs, 2 = fill f32 outputs. */
+/* Pair kernels (one wave per home cell). mode 0 = count, 1 = fill f64
+   outputs, 2 = fill f32 outputs. */
 hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
                         hipStream_t stream);
 hipError_t launch_set_params(SysGeom *geo, const SysGeom &g, int32_t *atom_base,

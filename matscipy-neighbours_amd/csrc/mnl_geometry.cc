@@ -67,6 +67,7 @@ bool make_sys_geom(const double origin[3], const double cell[9],
         g.pbc[k] = pbc[k] ? 1 : 0;
     }
     g.cutoff_sq = cutoff * cutoff;
+    for (int k = 0; k < 3; k++) g.rn[k] = 1.0 / g.n[k];
     g.atom_base = atom_base;
     g.natoms = natoms;
     g.cell_base = cell_base;

@@ -11,7 +11,9 @@
  *               cell-sorted per-atom records               (1 thread / atom)
  *   count       neighbours per atom                        (1 wave / cell)
  *   scan        counts -> int64 pair offsets, total on device
- *   fill        i, j, D, d, S at the offsets               (1 wave / cell)
+ *   fill        i, j, D, d, S at the offsets: the count's traversal again,
+ *               accepted pairs compacted in LDS and written 64 at a time
+ *                                                          (1 wave / cell)
  *
  * Floating point: compiled with contraction off and the reference's
  * operation order everywhere a double is produced, so the cell index, the
@@ -151,7 +153,8 @@ __device__ __forceinline__ void raw_cell_of(const double *__restrict__ r, int a,
 __global__ __launch_bounds__(kBlock) void k_bin_dense(
     const double *__restrict__ r, int32_t nat, const SysGeom *__restrict__ geo,
     const int32_t *__restrict__ atom_base, int32_t nsys, int4 *__restrict__ raw4,
-    int32_t *__restrict__ ticket, int32_t *__restrict__ cell_count) {
+    int32_t *__restrict__ ticket, int32_t *__restrict__ cell_count,
+    int32_t *__restrict__ cell_key) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nat) return;
     const SysGeom &g = geo[system_of(atom_base, nsys, a)];
@@ -163,6 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_dense(
         c[k] = g.pbc[k] ? wrap_idx(raw[k], g.n[k]) : clamp_idx(raw[k], g.n[k]);
     const int cell = g.cell_base + c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
     ticket[a] = atomicAdd(&cell_count[cell], 1);
+    atomicMin(&cell_key[cell], a);
     raw4[a] = make_int4(raw[0], raw[1], raw[2], cell);
 }
 
@@ -172,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_dense(
 __global__ __launch_bounds__(kBlock) void k_bin_hashed(
     const double *__restrict__ r, int32_t nat, const SysGeom *__restrict__ geo,
     int4 *__restrict__ raw4, int32_t *__restrict__ ticket, int64_t *__restrict__ hkey,
-    int64_t hmask, int32_t *__restrict__ hcount) {
+    int64_t hmask, int32_t *__restrict__ hcount, int32_t *__restrict__ cell_key) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nat) return;
     const SysGeom &g = geo[0];
@@ -195,7 +199,36 @@ __global__ __launch_bounds__(kBlock) void k_bin_hashed(
         h = (h + 1) & hmask;
     }
     ticket[a] = atomicAdd(&hcount[h], 1);
+    atomicMin(&cell_key[h], a);
     raw4[a] = make_int4(raw[0], raw[1], raw[2], static_cast<int>(h));
+}
+
+/* Memory order of the cells: by their smallest atom index, so that the
+   cell-sorted records follow the order of the input (cells neighbouring in
+   the input are neighbours in memory). The traversal does not depend on it:
+   a cell is found by its coordinates, and inside a cell atoms stay in
+   ascending index order. w[a] = atoms of a's cell if a is the cell's
+   smallest atom, else 0; its exclusive scan at a cell's smallest atom is the
+   cell's first slot. */
+__global__ __launch_bounds__(kBlock) void k_cell_weight(const int4 *__restrict__ raw4,
+                                                        int32_t nat,
+                                                        const int32_t *__restrict__ cell_key,
+                                                        const int32_t *__restrict__ cell_count,
+                                                        int32_t *__restrict__ w) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nat) return;
+    const int c = raw4[a].w;
+    w[a] = cell_key[c] == a ? cell_count[c] : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_first(const int32_t *__restrict__ cell_key,
+                                                       const int32_t *__restrict__ cell_count,
+                                                       int64_t ncells,
+                                                       const int32_t *__restrict__ wscan,
+                                                       int32_t *__restrict__ cell_first) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    cell_first[c] = cell_count[c] > 0 ? wscan[cell_key[c]] : 0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(const int4 *__restrict__ raw4,
@@ -213,9 +246,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const int4 *__restrict__ raw
    ascending atom order exactly like the reference's serial scatter
    (cell_list.cc:77-80). The cell-sorted record is then written as in
    neighbour_list.cc:147-166: rel = raw if periodic else clamped, rel_pos =
-   ((r - rel1 b1) - rel2 b2) - rel3 b3. The per-atom image count feeds the
-   shift: periodic img = (raw - wrap(raw)) / n (exact), non-periodic
-   img = trunc((raw - clamp(raw)) / n). */
+   ((r - rel1 b1) - rel2 b2) - rel3 b3. For the shift the record keeps, per
+   direction, the raw cell (periodic) or the image count
+   trunc((raw - clamp(raw)) / n) (non-periodic). */
 __global__ __launch_bounds__(kBlock) void k_rank_gather(
     const double *__restrict__ r, const int4 *__restrict__ raw4,
     const int32_t *__restrict__ tmp, int32_t nat, const int32_t *__restrict__ cell_first,
@@ -235,15 +268,15 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
     const int sys = system_of(atom_base, nsys, a);
     const SysGeom &g = geo[sys];
     const int raw[3] = {rc.x, rc.y, rc.z};
-    int rel[3], img[3];
+    int rel[3], base[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         if (g.pbc[k]) {
             rel[k] = raw[k];
-            img[k] = (raw[k] - wrap_idx(raw[k], g.n[k])) / g.n[k];
+            base[k] = raw[k];
         } else {
             rel[k] = clamp_idx(raw[k], g.n[k]);
-            img[k] = (raw[k] - rel[k]) / g.n[k];
+            base[k] = (raw[k] - rel[k]) / g.n[k];
         }
     }
     double p[3];
@@ -256,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
         p[k] = v;
     }
     spos[s] = make_double4(p[0], p[1], p[2], per_atom ? per_atom[a] : 0.0);
-    sinfo[s] = make_int4(img[0], img[1], img[2], a - g.atom_base);
+    sinfo[s] = make_int4(base[0], base[1], base[2], a - g.atom_base);
     if (stype) stype[s] = types ? types[a] : 0;
 }
 
@@ -416,18 +449,6 @@ __device__ __forceinline__ int4 load_const(const int4 *p, int i) {
     return make_int4(q[0], q[1], q[2], q[3]);
 }
 
-__device__ __forceinline__ int64_t readlane64(int64_t v, int lane) {
-    const int lo = __builtin_amdgcn_readlane(static_cast<int>(v), lane);
-    const int hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane);
-    return (static_cast<int64_t>(hi) << 32) | static_cast<uint32_t>(lo);
-}
-
-/* Chunks of 64 candidates whose accept masks the count pass keeps for the
-   fill (per home atom); a home cell with more chunks re-tests the rest in
-   the fill. */
-constexpr int kMaskChunks = 16;
-static_assert(kMaskChunks == kMaskChunksHost, "mask buffer sized by the host");
-
 }  // namespace
 /* The v_writelane intrinsic (no clang builtin spells it): returns `old` with
    lane `lane` replaced by the wave-uniform `val`. */
@@ -440,6 +461,12 @@ __device__ __forceinline__ uint32_t write_lane(uint32_t v, uint32_t s, int lane)
         mnl_llvm_writelane(static_cast<int>(s), lane, static_cast<int>(v)));
 }
 
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 /* Home cell decoded once per wave (all values wave-uniform). */
 struct HomeCell {
     int m, hb;             /* atoms in the cell, first cell-sorted atom */
@@ -450,6 +477,7 @@ struct HomeCell {
     int abase, cbase;
     double cutoff_sq;
     double b00, b01, b02, b10, b11, b12, b20, b21, b22;
+    double rn0, rn1, rn2;
 };
 
 template <bool HASHED>
@@ -495,6 +523,9 @@ __device__ __forceinline__ bool load_home(const PairArgs &A, int h, HomeCell &H)
     H.b20 = gp->bin[6];
     H.b21 = gp->bin[7];
     H.b22 = gp->bin[8];
+    H.rn0 = gp->rn[0];
+    H.rn1 = gp->rn[1];
+    H.rn2 = gp->rn[2];
     H.abase = gp->atom_base;
     H.cbase = gp->cell_base;
     return true;
@@ -503,22 +534,18 @@ __device__ __forceinline__ bool load_home(const PairArgs &A, int h, HomeCell &H)
 /* Segment table of one group of up to 64 neighbour-cell offsets, in the
    reference's traversal order. */
 struct SegLds {
-    double so0[kWave], so1[kWave], so2[kWave]; /* cell offset vector */
     int st[kWave];    /* first candidate index of each segment */
     int sb[kWave];    /* first cell-sorted atom of each segment */
-    int swr[kWave];   /* packed periodic wrap counts */
+    int sxyz[kWave];  /* packed cell offset (x, y, z) */
 };
 
 /* Builds the table of segments sg .. sg+nsg-1; returns the group's candidate
    count, and in *self_start the candidate index of the home cell's first atom
-   at zero offset (-1 if that segment is not in the group). Each segment's
-   offset vector is off = (z b3 + y b2) + x b1
-   (neighbour_visit.hh:67,73-74,80-81). */
+   at zero offset (-1 if that segment is not in the group). */
 template <bool HASHED>
 __device__ __forceinline__ int build_segments(const PairArgs &A, const HomeCell &H, int sg,
                                               int nsg, int lane, SegLds &S, int *self_start) {
-    int len = 0, begin = 0, pw = 0;
-    double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+    int len = 0, begin = 0, pxyz = 0;
     if (lane < nsg) {
         const int seg = sg + lane;
         const int x = seg % H.sx - H.nx;
@@ -526,50 +553,35 @@ __device__ __forceinline__ int build_segments(const PairArgs &A, const HomeCell 
         const int y = rest % H.sy - H.ny;
         const int z = rest / H.sy - H.nz;
         int cj0 = H.c0 + x, cj1 = H.c1 + y, cj2 = H.c2 + z;
-        int w0 = 0, w1 = 0, w2 = 0;
         bool ok = true;
-        if (H.p0) {
-            w0 = floor_div(cj0, H.n0);
-            cj0 -= w0 * H.n0;
-        } else {
+        if (H.p0)
+            cj0 -= floor_div(cj0, H.n0) * H.n0;
+        else
             ok = ok && cj0 >= 0 && cj0 < H.n0;
-        }
-        if (H.p1) {
-            w1 = floor_div(cj1, H.n1);
-            cj1 -= w1 * H.n1;
-        } else {
+        if (H.p1)
+            cj1 -= floor_div(cj1, H.n1) * H.n1;
+        else
             ok = ok && cj1 >= 0 && cj1 < H.n1;
-        }
-        if (H.p2) {
-            w2 = floor_div(cj2, H.n2);
-            cj2 -= w2 * H.n2;
-        } else {
+        if (H.p2)
+            cj2 -= floor_div(cj2, H.n2) * H.n2;
+        else
             ok = ok && cj2 >= 0 && cj2 < H.n2;
-        }
         if (ok) {
             if (HASHED)
                 hashed_slice(A, H.n0, H.n1, cj0, cj1, cj2, begin, len);
             else
                 dense_slice(A, H.cbase, H.n0, H.n1, cj0, cj1, cj2, begin, len);
         }
-        o0 = (z * H.b20 + y * H.b10) + x * H.b00;
-        o1 = (z * H.b21 + y * H.b11) + x * H.b01;
-        o2 = (z * H.b22 + y * H.b12) + x * H.b02;
-        pw = pack3(w0, w1, w2);
+        pxyz = pack3(x, y, z);
     }
     const int incl = wave_incl_scan(len);
     const int total = uniform(__shfl(incl, kWave - 1, kWave));
     const int ci = H.centre - sg;
     *self_start = (ci >= 0 && ci < nsg) ? uniform(__shfl(incl - len, ci, kWave)) : -1;
-    S.so0[lane] = o0;
-    S.so1[lane] = o1;
-    S.so2[lane] = o2;
     S.st[lane] = incl - len;
     S.sb[lane] = begin;
-    S.swr[lane] = pw;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    S.sxyz[lane] = pxyz;
+    wave_sync_lds();
     return total;
 }
 
@@ -586,49 +598,61 @@ __device__ __forceinline__ int last_at_or_below(const int *v, int n, int x) {
     return p;
 }
 
-/* Segment holding candidate c: the last one starting at or before c (empty
-   segments never win: the covering one comes later in order). */
-__device__ __forceinline__ int find_segment(const SegLds &S, int nsg, int c) {
-    return last_at_or_below(S.st, nsg, c);
-}
-
 /* One lane's candidate of the current chunk, as the test needs it. */
 struct Cand {
-    double4 pj;            /* bin-relative position (x = NaN for an idle lane) */
-    double o0, o1, o2;     /* cell offset vector */
+    double px, py, pz;     /* bin-relative position (px = NaN for an idle lane) */
+    double pw;             /* radius (per-atom cutoffs) */
+    double o0, o1, o2;     /* cell offset vector off = (z b3 + y b2) + x b1 */
     int tj;                /* type (per-type cutoffs) */
+    int j;                 /* atom index (system-local), fill only */
+    int c0, c1, c2;        /* shift numerator part x - raw_j (fill only) */
 };
 
-template <int CUT>
+/* Candidate c of the group's candidate space: its segment (the last one
+   starting at or before c; empty segments never win), cell-sorted index and
+   record. The offset vector follows neighbour_visit.hh:67,73-74,80-81. */
+template <int CUT, bool FILL>
 __device__ __forceinline__ void load_candidate(const PairArgs &A, const HomeCell &H,
                                                const SegLds &S, int nsg, int c, int total,
                                                Cand &C) {
     const bool cv = c < total;
-    const int p = find_segment(S, nsg, c);
+    const int p = last_at_or_below(S.st, nsg, c);
     const int sj = cv ? S.sb[p] + (c - S.st[p]) : H.hb;
-    C.o0 = S.so0[p];
-    C.o1 = S.so1[p];
-    C.o2 = S.so2[p];
-    C.pj = A.spos[sj];
-    if (!cv) C.pj.x = __builtin_nan("");
+    int x, y, z;
+    unpack3(S.sxyz[p], x, y, z);
+    C.o0 = (z * H.b20 + y * H.b10) + x * H.b00;
+    C.o1 = (z * H.b21 + y * H.b11) + x * H.b01;
+    C.o2 = (z * H.b22 + y * H.b12) + x * H.b02;
+    const double4 pj = A.spos[sj];
+    C.px = cv ? pj.x : __builtin_nan("");
+    C.py = pj.y;
+    C.pz = pj.z;
+    C.pw = pj.w;
     C.tj = CUT == CUT_PER_TYPE ? A.stype[sj] : 0;
+    if (FILL) {
+        const int4 bj = A.sinfo[sj];
+        C.j = bj.w;
+        C.c0 = x - bj.x;
+        C.c1 = y - bj.y;
+        C.c2 = z - bj.z;
+    }
 }
 
-/* The distance test of neighbour_visit.hh:89-109 for home atom si at pi:
+/* The distance test of neighbour_visit.hh:89-109 for a home atom at pi:
    dr = (rel_j - rel_i) + off, r2 = (dx^2 + dy^2) + dz^2, strict r2 < rc^2,
    then the per-atom (sum of radii) or per-type (matrix entry when both types
-   are in range) cutoff. The self pair is removed from the ballot by the
-   caller. */
+   are in range) cutoff. dr is returned for the fill. */
 template <int CUT>
 __device__ __forceinline__ bool pair_test(const PairArgs &A, const HomeCell &H, const Cand &C,
-                                          int si, const double4 &pi) {
-    const double d0 = (C.pj.x - pi.x) + C.o0;
-    const double d1 = (C.pj.y - pi.y) + C.o1;
-    const double d2 = (C.pj.z - pi.z) + C.o2;
+                                          int si, const double4 &pi, double &d0, double &d1,
+                                          double &d2) {
+    d0 = (C.px - pi.x) + C.o0;
+    d1 = (C.py - pi.y) + C.o1;
+    d2 = (C.pz - pi.z) + C.o2;
     const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
     bool ok = r2 < H.cutoff_sq;
     if (CUT == CUT_PER_ATOM) {
-        const double cc = pi.w + C.pj.w;
+        const double cc = pi.w + C.pw;
         ok = ok & (r2 < cc * cc);
     } else if (CUT == CUT_PER_TYPE) {
         const int ti = as_const(A.stype)[si];
@@ -641,19 +665,6 @@ __device__ __forceinline__ bool pair_test(const PairArgs &A, const HomeCell &H, 
     return ok;
 }
 
-/* Ballot of the test of the chunk starting at cb against home atom t of the
-   group, self pair removed: the home atom sits at candidate
-   self_start + grp0 + t of the zero-offset segment (neighbour_visit.hh:86). */
-template <int CUT>
-__device__ __forceinline__ uint64_t chunk_mask(const PairArgs &A, const HomeCell &H,
-                                               const Cand &C, int si, int self_c, int cb) {
-    const double4 pi = load_const(A.spos, si);
-    uint64_t mask = __builtin_amdgcn_ballot_w64(pair_test<CUT>(A, H, C, si, pi));
-    const int off = self_c - cb;
-    if (off >= 0 && off < kWave) mask &= ~(1ull << off);
-    return mask;
-}
-
 /*
  * Count pass. One wave per home cell. The home cell's neighbour cells, in
  * the reference's traversal order (z outer, y, x inner, each -n..n;
@@ -661,10 +672,9 @@ __device__ __forceinline__ uint64_t chunk_mask(const PairArgs &A, const HomeCell
  * space: lanes take 64 consecutive candidates (coalesced loads, kept in
  * registers) and the wave loops over the home atoms, testing all 64
  * candidates against one home atom per step (the home atom's record comes
- * by a scalar load). The ballot of atom t is written into lane t; its
- * popcount is the count, and for the first kMaskChunks chunks it is stored
- * as the (chunk, atom) accept mask so that the fill does not test again.
- * Home atoms are taken in groups of 64.
+ * by a scalar load). The ballot of atom t, self pair removed
+ * (neighbour_visit.hh:86), is written into lane t; its popcount is the
+ * count. Home atoms are taken in groups of 64.
  */
 template <int CUT, bool HASHED>
 __device__ __forceinline__ void count_cell(const PairArgs &A, int h, SegLds &S) {
@@ -674,26 +684,27 @@ __device__ __forceinline__ void count_cell(const PairArgs &A, int h, SegLds &S) 
     for (int grp0 = 0; grp0 < H.m; grp0 += kWave) {
         const int mg = uniform(min(kWave, H.m - grp0));
         int acc = 0;
-        int k = 0;
         for (int sg = 0; sg < H.nseg; sg += kWave) {
             const int nsg = min(kWave, H.nseg - sg);
             int self_start;
             const int total = build_segments<HASHED>(A, H, sg, nsg, lane, S, &self_start);
             const int self_base = self_start >= 0 ? self_start + grp0 : -(1 << 30);
-            for (int cb = 0; cb < total; cb += kWave, k++) {
+            for (int cb = 0; cb < total; cb += kWave) {
                 Cand C;
-                load_candidate<CUT>(A, H, S, nsg, cb + lane, total, C);
+                load_candidate<CUT, false>(A, H, S, nsg, cb + lane, total, C);
+                const int c = cb + lane;
                 uint32_t mlo = 0, mhi = 0;
                 for (int t = 0; t < mg; t++) {
                     const int si = H.hb + grp0 + t;
-                    const uint64_t mask = chunk_mask<CUT>(A, H, C, si, self_base + t, cb);
+                    const double4 pi = load_const(A.spos, si);
+                    double d0, d1, d2;
+                    const bool ok = pair_test<CUT>(A, H, C, si, pi, d0, d1, d2) &
+                                    (c != self_base + t);
+                    const uint64_t mask = __builtin_amdgcn_ballot_w64(ok);
                     mlo = write_lane(mlo, static_cast<uint32_t>(mask), t);
                     mhi = write_lane(mhi, static_cast<uint32_t>(mask >> 32), t);
                 }
                 acc += __popc(mlo) + __popc(mhi);
-                if (k < kMaskChunks && lane < mg)
-                    A.masks[static_cast<int64_t>(k) * A.mask_stride + H.hb + grp0 + lane] =
-                        (static_cast<uint64_t>(mhi) << 32) | mlo;
             }
             __builtin_amdgcn_wave_barrier();
         }
@@ -701,59 +712,106 @@ __device__ __forceinline__ void count_cell(const PairArgs &A, int h, SegLds &S) 
     }
 }
 
-/* Fill-pass LDS of one wave: segment table, home-atom table of the current
-   group, accept masks of the current chunk window ([chunk][atom]), the
-   window's per-atom pair prefix and each atom's running output slot.
-   Structure of arrays: consecutive lanes touch consecutive banks. */
-constexpr int kWindow = 8;
-struct FillLds {
-    SegLds seg;
-    double hx[kWave], hy[kWave], hz[kWave];
-    int hi[kWave], hm0[kWave], hm1[kWave], hm2[kWave];
-    uint64_t msk[kWindow][kWave];
-    int cpre[kWindow][kWave]; /* pairs of atom t in the window's chunks before kk */
-    int pre[kWave];
-    int64_t cur[kWave];
-};
-
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+template <int CUT, bool HASHED>
+__global__ __launch_bounds__(kBlock) void k_count(PairArgs A) {
+    __shared__ SegLds s_lds[kWavesPerBlock];
+    const int wib = uniform(threadIdx.x / kWave);
+    const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
+    const int h = blk * kWavesPerBlock + wib;
+    if (h >= A.home_cells) return;
+    count_cell<CUT, HASHED>(A, h, s_lds[wib]);
 }
 
-/* Position of the r-th (from 0) set bit of m; r < popcount(m). */
-__device__ __forceinline__ int select_bit(uint64_t m, int r) {
-    const uint32_t lo = static_cast<uint32_t>(m), hi = static_cast<uint32_t>(m >> 32);
-    const int clo = __popc(lo);
-    const bool up = r >= clo;
-    uint32_t w = up ? hi : lo;
-    r -= up ? clo : 0;
-    int pos = up ? 32 : 0;
-#pragma unroll
-    for (int width = 16; width > 0; width >>= 1) {
-        const int c = __popc(w & ((1u << width) - 1u));
-        const bool u = r >= c;
-        r -= u ? c : 0;
-        w = u ? (w >> width) : w;
-        pos += u ? width : 0;
+/* ------------------------------------------------------------- fill pass */
+
+/* Candidate chunks held in registers while the home atoms are swept, and
+   the ring of accepted pairs waiting for a full 64-lane store. */
+constexpr int kFillChunks = 4;
+constexpr int kRing = 128;
+
+/* Shift along one direction: periodic S = (raw_i + x - raw_j) / n, an exact
+   quotient (raw = wrapped cell + n * image on both sides, and the visited
+   cell is c_i + x wrapped), taken as rint((double)num * (1/n)), exact for
+   |S| < 2^51; non-periodic S = image count of i. This is the reference's
+   truncating (raw_i - rel_j + x) / n (neighbour_visit.hh:111-114). */
+__device__ __forceinline__ int shift_of(int periodic, int bi, int cj, double rn) {
+    return periodic ? static_cast<int>(rint(static_cast<double>(bi + cj) * rn)) : bi;
+}
+
+/* Write one pair: D = dr as the test computed it, d = sqrt(r2) with r2 in
+   the test's operation order (the bits the test saw). */
+template <int MODE>
+__device__ __forceinline__ void write_pair(const PairArgs &A, int64_t pos, int i, int j,
+                                           double dx, double dy, double dz, int s0, int s1,
+                                           int s2) {
+    if (pos >= A.capacity) return;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (A.out_i) A.out_i[pos] = i;
+    if (A.out_j) A.out_j[pos] = j;
+    if (MODE == MODE_FILL64) {
+        if (A.out_D) {
+            double *D = static_cast<double *>(A.out_D) + 3 * pos;
+            D[0] = dx;
+            D[1] = dy;
+            D[2] = dz;
+        }
+        if (A.out_d) static_cast<double *>(A.out_d)[pos] = sqrt(r2);
+    } else {
+        if (A.out_D) {
+            float *D = static_cast<float *>(A.out_D) + 3 * pos;
+            D[0] = static_cast<float>(dx);
+            D[1] = static_cast<float>(dy);
+            D[2] = static_cast<float>(dz);
+        }
+        if (A.out_d) static_cast<float *>(A.out_d)[pos] = static_cast<float>(sqrt(r2));
     }
-    return pos;
+    if (A.out_S) {
+        int32_t *S = A.out_S + 3 * pos;
+        S[0] = s0;
+        S[1] = s1;
+        S[2] = s2;
+    }
+}
+
+/* Fill-pass LDS of one wave: segment table, the home atoms of the current
+   group (output row, image/raw cell, index) and the ring of accepted pairs
+   (dr, j, shift numerator part, atom and rank in its row). */
+struct FillLds {
+    SegLds seg;
+    int64_t hpos[kWave];
+    int hbx[kWave], hby[kWave], hbz[kWave], hi[kWave];
+    double rdx[kRing], rdy[kRing], rdz[kRing];
+    int rj[kRing], rc0[kRing], rc1[kRing], rc2[kRing], rtk[kRing];
+};
+
+/* Write n (<= 64) ring entries from `tail` on, one per lane. */
+template <int MODE>
+__device__ __forceinline__ void flush_ring(const PairArgs &A, const HomeCell &H, FillLds &L,
+                                           int tail, int n) {
+    const int lane = lane_id();
+    wave_sync_lds();
+    if (lane < n) {
+        const int e = (tail + lane) & (kRing - 1);
+        const int tk = L.rtk[e];
+        const int t = tk & (kWave - 1);
+        const int64_t pos = L.hpos[t] + (tk >> 6);
+        write_pair<MODE>(A, pos, L.hi[t], L.rj[e], L.rdx[e], L.rdy[e], L.rdz[e],
+                         shift_of(H.p0, L.hbx[t], L.rc0[e], H.rn0),
+                         shift_of(H.p1, L.hby[t], L.rc1[e], H.rn1),
+                         shift_of(H.p2, L.hbz[t], L.rc2[e], H.rn2));
+    }
+    wave_sync_lds();
 }
 
 /*
- * Fill pass. Same traversal and chunking as the count; the accept mask of
- * every (chunk, home atom) comes from the count pass (or is re-tested beyond
- * kMaskChunks). Per window of up to kWindow chunks the masks go to LDS, a
- * wave scan of the per-atom popcounts gives each atom's first pair in the
- * window, and the window's pairs - ordered by atom, then chunk, then lane,
- * i.e. by output slot - are written 64 per step, one per lane: the lane
- * finds its atom (binary search of the prefixes), its chunk (running
- * popcount) and its candidate (rank select in the mask), looks the candidate
- * up again and writes i, j, D, d, S at the atom's running slot. One atom's
- * pairs take consecutive slots in candidate order - the reference's output
- * order - and consecutive lanes write consecutive slots, so every atom's
- * output segment is written in one short coalesced burst.
+ * Fill pass, same traversal as the count. Per window of up to kFillChunks
+ * candidate chunks (held in registers) the wave sweeps the home atoms and,
+ * for each atom, the window's chunks in order: the test is repeated and the
+ * accepted lanes append (dr, j, shift part, atom, rank) to an LDS ring - atom
+ * by atom, in candidate order, i.e. in output order within each atom. Every
+ * time the ring holds 64 pairs they are written with one full-width store
+ * per output array, at the atom's row offset plus the pair's rank. Per-atom
+ * ranks continue across windows, so an atom's row fills front to back.
  */
 template <int MODE, int CUT, bool HASHED>
 __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L) {
@@ -763,139 +821,82 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L) 
     for (int grp0 = 0; grp0 < H.m; grp0 += kWave) {
         const int mg = uniform(min(kWave, H.m - grp0));
         if (lane < mg) {
-            const int si = H.hb + grp0 + lane;
-            const double4 pp = A.spos[si];
-            const int4 inf = A.sinfo[si];
-            L.hx[lane] = pp.x;
-            L.hy[lane] = pp.y;
-            L.hz[lane] = pp.z;
+            const int4 inf = A.sinfo[H.hb + grp0 + lane];
+            L.hpos[lane] = A.offsets[H.abase + inf.w];
+            L.hbx[lane] = inf.x;
+            L.hby[lane] = inf.y;
+            L.hbz[lane] = inf.z;
             L.hi[lane] = inf.w;
-            L.hm0[lane] = inf.x;
-            L.hm1[lane] = inf.y;
-            L.hm2[lane] = inf.z;
-            L.cur[lane] = A.offsets[H.abase + inf.w];
         }
-        int kbase = 0;
+        wave_sync_lds();
+        int acc = 0;     /* lane t: pairs of atom t emitted so far */
+        int head = 0;    /* ring positions (uniform) */
+        int tail = 0;
         for (int sg = 0; sg < H.nseg; sg += kWave) {
             const int nsg = min(kWave, H.nseg - sg);
             int self_start;
             const int total = build_segments<HASHED>(A, H, sg, nsg, lane, L.seg, &self_start);
-            const int nch = (total + kWave - 1) / kWave;
             const int self_base = self_start >= 0 ? self_start + grp0 : -(1 << 30);
-            for (int kb0 = 0; kb0 < nch; kb0 += kWindow) {
-                const int nb = min(kWindow, nch - kb0);
-                /* 1. the window's accept masks, [chunk][atom] */
-                int cnt = 0;
-                for (int kk = 0; kk < nb; kk++) {
-                    const int k = kbase + kb0 + kk;
-                    uint64_t mk = 0;
-                    if (k < kMaskChunks) {
-                        if (lane < mg)
-                            mk = A.masks[static_cast<int64_t>(k) * A.mask_stride + H.hb + grp0 +
-                                         lane];
-                    } else {
-                        Cand C;
-                        const int cb = (kb0 + kk) * kWave;
-                        load_candidate<CUT>(A, H, L.seg, nsg, cb + lane, total, C);
-                        uint32_t mlo = 0, mhi = 0;
-                        for (int t = 0; t < mg; t++) {
-                            const uint64_t m =
-                                chunk_mask<CUT>(A, H, C, H.hb + grp0 + t, self_base + t, cb);
-                            mlo = write_lane(mlo, static_cast<uint32_t>(m), t);
-                            mhi = write_lane(mhi, static_cast<uint32_t>(m >> 32), t);
-                        }
-                        mk = (static_cast<uint64_t>(mhi) << 32) | mlo;
-                    }
-                    L.msk[kk][lane] = mk;
-                    L.cpre[kk][lane] = cnt;
-                    cnt += __popcll(mk);
-                }
-                /* 2. each atom's first pair in the window */
-                const int incl = wave_incl_scan(cnt);
-                const int tw = uniform(__shfl(incl, kWave - 1, kWave));
-                L.pre[lane] = incl - cnt;
-                wave_sync_lds();
-                /* 3. the window's pairs, 64 per step */
-                for (int qb = 0; qb < tw; qb += kWave) {
-                    const int q = qb + lane;
-                    if (q < tw) {
-                        const int t = last_at_or_below(L.pre, mg, q);
-                        const int rank = q - L.pre[t];
-                        /* chunk: the last whose running count is <= rank */
-                        int kk = 0;
+            for (int w0 = 0; w0 < total; w0 += kWave * kFillChunks) {
+                Cand C[kFillChunks];
 #pragma unroll
-                        for (int j = 1; j < kWindow; j++)
-                            kk += ((j < nb) & (L.cpre[j][t] <= rank)) ? 1 : 0;
-                        const uint64_t mk = L.msk[kk][t];
-                        const int r = rank - L.cpre[kk][t];
-                        const int c = (kb0 + kk) * kWave + select_bit(mk, r);
-                        const int p = find_segment(L.seg, nsg, c);
-                        const int sj = L.seg.sb[p] + (c - L.seg.st[p]);
-                        const int64_t pos = L.cur[t] + rank;
-                        const double4 pj = A.spos[sj];
-                        const int4 ij = A.sinfo[sj];
-                        if (pos < A.capacity) {
-                            /* D = (pj - pi) + off and r2 in the traversal's
-                               operation order: the bits the test saw */
-                            const double dx = (pj.x - L.hx[t]) + L.seg.so0[p];
-                            const double dy = (pj.y - L.hy[t]) + L.seg.so1[p];
-                            const double dz = (pj.z - L.hz[t]) + L.seg.so2[p];
-                            const double r2 = dx * dx + dy * dy + dz * dz;
-                            if (A.out_i) A.out_i[pos] = L.hi[t];
-                            if (A.out_j) A.out_j[pos] = ij.w;
-                            if (MODE == MODE_FILL64) {
-                                if (A.out_D) {
-                                    double *D = static_cast<double *>(A.out_D) + 3 * pos;
-                                    D[0] = dx;
-                                    D[1] = dy;
-                                    D[2] = dz;
-                                }
-                                if (A.out_d) static_cast<double *>(A.out_d)[pos] = sqrt(r2);
-                            } else {
-                                if (A.out_D) {
-                                    float *D = static_cast<float *>(A.out_D) + 3 * pos;
-                                    D[0] = static_cast<float>(dx);
-                                    D[1] = static_cast<float>(dy);
-                                    D[2] = static_cast<float>(dz);
-                                }
-                                if (A.out_d)
-                                    static_cast<float *>(A.out_d)[pos] = static_cast<float>(sqrt(r2));
+                for (int k = 0; k < kFillChunks; k++)
+                    if (w0 + k * kWave < total)
+                        load_candidate<CUT, true>(A, H, L.seg, nsg, w0 + k * kWave + lane,
+                                                  total, C[k]);
+                for (int t = 0; t < mg; t++) {
+                    const int si = H.hb + grp0 + t;
+                    const double4 pi = load_const(A.spos, si);
+#pragma unroll
+                    for (int k = 0; k < kFillChunks; k++) {
+                        const int cb = w0 + k * kWave;
+                        if (cb >= total) break;
+                        double d0, d1, d2;
+                        const bool ok = pair_test<CUT>(A, H, C[k], si, pi, d0, d1, d2) &
+                                        (cb + lane != self_base + t);
+                        const uint64_t mask = __builtin_amdgcn_ballot_w64(ok);
+                        if (mask) {
+                            const int r = mbcnt64(mask);
+                            const int rank = __builtin_amdgcn_readlane(acc, t) + r;
+                            if (ok) {
+                                const int e = (head + r) & (kRing - 1);
+                                L.rdx[e] = d0;
+                                L.rdy[e] = d1;
+                                L.rdz[e] = d2;
+                                L.rj[e] = C[k].j;
+                                L.rc0[e] = C[k].c0;
+                                L.rc1[e] = C[k].c1;
+                                L.rc2[e] = C[k].c2;
+                                L.rtk[e] = t | (rank << 6);
                             }
-                            if (A.out_S) {
-                                /* S = img_i + w - img_j (periodic), img_i
-                                   (non-periodic) */
-                                int w0, w1, w2;
-                                unpack3(L.seg.swr[p], w0, w1, w2);
-                                int32_t *S = A.out_S + 3 * pos;
-                                S[0] = L.hm0[t] + (H.p0 ? w0 - ij.x : 0);
-                                S[1] = L.hm1[t] + (H.p1 ? w1 - ij.y : 0);
-                                S[2] = L.hm2[t] + (H.p2 ? w2 - ij.z : 0);
+                            const int pc = static_cast<int>(__popcll(mask));
+                            acc = static_cast<int>(write_lane(
+                                static_cast<uint32_t>(acc),
+                                static_cast<uint32_t>(__builtin_amdgcn_readlane(acc, t) + pc),
+                                t));
+                            head += pc;
+                            if (head - tail >= kWave) {
+                                flush_ring<MODE>(A, H, L, tail, kWave);
+                                tail += kWave;
                             }
                         }
                     }
                 }
-                /* 4. advance the atoms' slots past this window */
-                wave_sync_lds();
-                if (lane < mg) L.cur[lane] += cnt;
-                wave_sync_lds();
             }
-            kbase += nch;
+            __builtin_amdgcn_wave_barrier();
         }
+        if (head > tail) flush_ring<MODE>(A, H, L, tail, head - tail);
     }
 }
 
 template <int MODE, int CUT, bool HASHED>
-__global__ __launch_bounds__(kBlock) void k_pairs(PairArgs A) {
-    using Lds = typename std::conditional<MODE == MODE_COUNT, SegLds, FillLds>::type;
-    __shared__ Lds s_lds[kWavesPerBlock];
+__global__ __launch_bounds__(kBlock) void k_fill(PairArgs A) {
+    __shared__ FillLds s_lds[kWavesPerBlock];
     const int wib = uniform(threadIdx.x / kWave);
     const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
     const int h = blk * kWavesPerBlock + wib;
     if (h >= A.home_cells) return;
-    if constexpr (MODE == MODE_COUNT)
-        count_cell<CUT, HASHED>(A, h, s_lds[wib]);
-    else
-        fill_cell<MODE, CUT, HASHED>(A, h, s_lds[wib]);
+    fill_cell<MODE, CUT, HASHED>(A, h, s_lds[wib]);
 }
 
 /* Device-side parameter set-up (one thread): keeps the whole build free of
@@ -1052,19 +1053,37 @@ hipError_t launch_fill_i64(int64_t *p, int64_t n, int64_t v, hipStream_t stream)
 
 hipError_t launch_bin_dense(const double *r, int32_t nat, const SysGeom *geo,
                             const int32_t *atom_base, int32_t nsys, int4 *raw4,
-                            int32_t *ticket, int32_t *cell_count, hipStream_t stream) {
+                            int32_t *ticket, int32_t *cell_count, int32_t *cell_key,
+                            hipStream_t stream) {
     if (nat <= 0) return hipSuccess;
     k_bin_dense<<<grid_for(nat), kBlock, 0, stream>>>(r, nat, geo, atom_base, nsys, raw4,
-                                                      ticket, cell_count);
+                                                      ticket, cell_count, cell_key);
     return hipGetLastError();
 }
 
 hipError_t launch_bin_hashed(const double *r, int32_t nat, const SysGeom *geo, int4 *raw4,
                              int32_t *ticket, int64_t *hkey, int64_t hmask,
-                             int32_t *hcount, hipStream_t stream) {
+                             int32_t *hcount, int32_t *cell_key, hipStream_t stream) {
     if (nat <= 0) return hipSuccess;
     k_bin_hashed<<<grid_for(nat), kBlock, 0, stream>>>(r, nat, geo, raw4, ticket, hkey,
-                                                       hmask, hcount);
+                                                       hmask, hcount, cell_key);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_order(const int4 *raw4, int32_t nat, const int32_t *cell_key,
+                             const int32_t *cell_count, int64_t ncells, int32_t *w,
+                             int32_t *wscan, int64_t *partial, int32_t *cell_first,
+                             hipStream_t stream) {
+    if (nat <= 0) {
+        if (ncells > 0)
+            k_fill_i32<<<grid_for(ncells), kBlock, 0, stream>>>(cell_first, ncells, 0);
+        return hipGetLastError();
+    }
+    k_cell_weight<<<grid_for(nat), kBlock, 0, stream>>>(raw4, nat, cell_key, cell_count, w);
+    hipError_t e = launch_scan_i32_to_i32(w, nat, wscan, partial, nullptr, stream);
+    if (e != hipSuccess) return e;
+    k_cell_first<<<grid_for(ncells), kBlock, 0, stream>>>(cell_key, cell_count, ncells, wscan,
+                                                          cell_first);
     return hipGetLastError();
 }
 
@@ -1122,42 +1141,51 @@ hipError_t launch_scan_i32_to_i64(const int32_t *in, int64_t n, int64_t *out,
     return scan_impl<int64_t>(in, n, out, partial, total, true, stream);
 }
 
+template <int CUT, bool HASHED>
+static void launch_count_t(const PairArgs &a, unsigned nblocks, hipStream_t stream) {
+    k_count<CUT, HASHED><<<nblocks, kBlock, 0, stream>>>(a);
+}
+
 template <int MODE, int CUT, bool HASHED>
-static void launch_pairs_t(const PairArgs &a, unsigned nblocks, hipStream_t stream) {
-    k_pairs<MODE, CUT, HASHED><<<nblocks, kBlock, 0, stream>>>(a);
+static void launch_fill_t(const PairArgs &a, unsigned nblocks, hipStream_t stream) {
+    k_fill<MODE, CUT, HASHED><<<nblocks, kBlock, 0, stream>>>(a);
 }
 
-template <int MODE, int CUT>
-static void launch_pairs_h(bool hashed, const PairArgs &a, unsigned nb, hipStream_t s) {
-    if (hashed)
-        launch_pairs_t<MODE, CUT, true>(a, nb, s);
-    else
-        launch_pairs_t<MODE, CUT, false>(a, nb, s);
-}
-
-template <int MODE>
-static void launch_pairs_c(int cut, bool hashed, const PairArgs &a, unsigned nb, hipStream_t s) {
-    if (cut == CUT_PER_ATOM)
-        launch_pairs_h<MODE, CUT_PER_ATOM>(hashed, a, nb, s);
-    else if (cut == CUT_PER_TYPE)
-        launch_pairs_h<MODE, CUT_PER_TYPE>(hashed, a, nb, s);
-    else
-        launch_pairs_h<MODE, CUT_GLOBAL>(hashed, a, nb, s);
+template <int CUT>
+static void launch_pairs_h(int mode, bool hashed, const PairArgs &a, unsigned nb,
+                           hipStream_t s) {
+    if (mode == MODE_COUNT) {
+        if (hashed)
+            launch_count_t<CUT, true>(a, nb, s);
+        else
+            launch_count_t<CUT, false>(a, nb, s);
+    } else if (mode == MODE_FILL64) {
+        if (hashed)
+            launch_fill_t<MODE_FILL64, CUT, true>(a, nb, s);
+        else
+            launch_fill_t<MODE_FILL64, CUT, false>(a, nb, s);
+    } else {
+        if (hashed)
+            launch_fill_t<MODE_FILL32, CUT, true>(a, nb, s);
+        else
+            launch_fill_t<MODE_FILL32, CUT, false>(a, nb, s);
+    }
 }
 
 hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
                         hipStream_t stream) {
-    if (args.home_cells <= 0) return hipSuccess;
-    const unsigned nb =
-        static_cast<unsigned>((args.home_cells + kWavesPerBlock - 1) / kWavesPerBlock);
+    /* one wave per home cell */
+    const int64_t waves = static_cast<int64_t>(args.home_cells);
+    if (waves <= 0) return hipSuccess;
+    const unsigned nb = static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock);
     PairArgs a = args;
     a.xcd_blocks = static_cast<int32_t>(nb);
-    if (mode == MODE_COUNT)
-        launch_pairs_c<MODE_COUNT>(cut, hashed, a, nb, stream);
-    else if (mode == MODE_FILL64)
-        launch_pairs_c<MODE_FILL64>(cut, hashed, a, nb, stream);
+    if (cut == CUT_PER_ATOM)
+        launch_pairs_h<CUT_PER_ATOM>(mode, hashed, a, nb, stream);
+    else if (cut == CUT_PER_TYPE)
+        launch_pairs_h<CUT_PER_TYPE>(mode, hashed, a, nb, stream);
     else
-        launch_pairs_c<MODE_FILL32>(cut, hashed, a, nb, stream);
+        launch_pairs_h<CUT_GLOBAL>(mode, hashed, a, nb, stream);
     return hipGetLastError();
 }
 
