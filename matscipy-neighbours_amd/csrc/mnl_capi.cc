@@ -122,7 +122,7 @@ struct mnl_plan {
     /* workspaces */
     DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, counts,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
-        htypes, cell_partial, cell_key;
+        htypes, cell_partial;
     int64_t *host_total = nullptr; /* pinned */
     /* state of the last count */
     bool counted = false;
@@ -192,7 +192,6 @@ int ensure_workspaces(mnl_plan *p, hipStream_t stream, int32_t nat, int64_t cell
         {&p->offsets, (n + 1) * sizeof(int64_t)},
         {&p->cell_count, static_cast<size_t>(std::max<int64_t>(cells, 1)) * sizeof(int32_t)},
         {&p->cell_first, static_cast<size_t>(std::max<int64_t>(cells, 1)) * sizeof(int32_t)},
-        {&p->cell_key, static_cast<size_t>(std::max<int64_t>(cells, 1)) * sizeof(int32_t)},
         {&p->partial, static_cast<size_t>(nparts) * sizeof(int64_t)},
         {&p->geo, sizeof(SysGeom) * static_cast<size_t>(nsys)},
         {&p->atom_base, (static_cast<size_t>(nsys) + 1) * sizeof(int32_t)},
@@ -225,25 +224,20 @@ int enqueue_build(mnl_plan *p, hipStream_t stream, const double *positions,
     const int64_t cells = p->home_cells;
     const int32_t nat = p->nat;
     MNL_HIP(launch_fill_i32(p->cell_count.as<int32_t>(), cells, 0, stream));
-    MNL_HIP(launch_fill_i32(p->cell_key.as<int32_t>(), cells, INT32_MAX, stream));
     if (hashed) {
         MNL_HIP(launch_fill_i64(p->hkey.as<int64_t>(), cells, -1, stream));
         MNL_HIP(launch_bin_hashed(positions, nat, p->geo.as<SysGeom>(), p->raw4.as<int4>(),
                                   p->ticket.as<int32_t>(), p->hkey.as<int64_t>(), p->hmask,
-                                  p->cell_count.as<int32_t>(), p->cell_key.as<int32_t>(),
-                                  stream));
+                                  p->cell_count.as<int32_t>(), stream));
     } else {
         MNL_HIP(launch_bin_dense(positions, nat, p->geo.as<SysGeom>(),
                                  p->atom_base.as<int32_t>(), p->nsys, p->raw4.as<int4>(),
                                  p->ticket.as<int32_t>(), p->cell_count.as<int32_t>(),
-                                 p->cell_key.as<int32_t>(), stream));
+                                 stream));
     }
-    /* slices laid out by each cell's smallest atom (tmp and counts hold the
-       weights and their scan until the scatter and the count overwrite them) */
-    MNL_HIP(launch_cell_order(p->raw4.as<int4>(), nat, p->cell_key.as<int32_t>(),
-                              p->cell_count.as<int32_t>(), cells, p->tmp.as<int32_t>(),
-                              p->counts.as<int32_t>(), p->partial.as<int64_t>(),
-                              p->cell_first.as<int32_t>(), stream));
+    MNL_HIP(launch_scan_i32_to_i32(p->cell_count.as<int32_t>(), cells,
+                                   p->cell_first.as<int32_t>(), p->partial.as<int64_t>(),
+                                   nullptr, stream));
     MNL_HIP(launch_scatter(p->raw4.as<int4>(), p->ticket.as<int32_t>(), nat,
                            p->cell_first.as<int32_t>(), p->tmp.as<int32_t>(), stream));
     MNL_HIP(launch_rank_gather(positions, p->raw4.as<int4>(), p->tmp.as<int32_t>(), nat,
@@ -578,7 +572,7 @@ void mnl_plan_destroy(mnl_plan *p) {
                       &p->stype,     &p->cell_count, &p->cell_first, &p->counts, &p->offsets,
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
                       &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
-                      &p->cell_partial, &p->cell_key};
+                      &p->cell_partial};
     for (DevBuf *b : bufs) b->release();
     if (p->host_total) (void)hipHostFree(p->host_total);
     delete p;

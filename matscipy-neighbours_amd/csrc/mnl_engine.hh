@@ -9,9 +9,7 @@
  *   sinfo  int4   [N]  cell-sorted: per direction the raw cell (periodic) or
  *                      the image count (non-periodic), + atom index (system-local)
  *   stype  int32  [N]  cell-sorted atom type (per-type cutoffs only)
- *   cell_count / cell_first int32 [C] dense CSR (or hash slots when hashed);
- *                      slices laid out by each cell's smallest atom index
- *   cell_key int32 [C] smallest atom index of each cell
+ *   cell_count / cell_first int32 [C] dense CSR (or hash slots when hashed)
  *   counts int32  [N]  neighbours per atom, original order
  *   offsets int64 [N+1] exclusive scan of counts (CSR row pointers)
  * A "system" is one periodic cell with its atoms; the plain path is a batch
@@ -91,19 +89,11 @@ hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t stream)
 hipError_t launch_fill_i64(int64_t *p, int64_t n, int64_t v, hipStream_t stream);
 hipError_t launch_bin_dense(const double *r, int32_t nat, const SysGeom *geo,
                             const int32_t *atom_base, int32_t nsys, int4 *raw4,
-                            int32_t *ticket, int32_t *cell_count, int32_t *cell_key,
+                            int32_t *ticket, int32_t *cell_count,
                             hipStream_t stream);
 hipError_t launch_bin_hashed(const double *r, int32_t nat, const SysGeom *geo,
                              int4 *raw4, int32_t *ticket, int64_t *hkey,
-                             int64_t hmask, int32_t *hcount, int32_t *cell_key,
-                             hipStream_t stream);
-/* Cell slices ordered by each cell's smallest atom index (cell_key, from the
-   binning's atomicMin): cell_first for every cell. w and wscan need nat
-   entries each, partial scan_partial_size(nat). */
-hipError_t launch_cell_order(const int4 *raw4, int32_t nat, const int32_t *cell_key,
-                             const int32_t *cell_count, int64_t ncells, int32_t *w,
-                             int32_t *wscan, int64_t *partial, int32_t *cell_first,
-                             hipStream_t stream);
+                             int64_t hmask, int32_t *hcount, hipStream_t stream);
 hipError_t launch_scatter(const int4 *raw4, const int32_t *ticket, int32_t nat,
                           const int32_t *cell_first, int32_t *tmp,
                           hipStream_t stream);

@@ -153,8 +153,7 @@ __device__ __forceinline__ void raw_cell_of(const double *__restrict__ r, int a,
 __global__ __launch_bounds__(kBlock) void k_bin_dense(
     const double *__restrict__ r, int32_t nat, const SysGeom *__restrict__ geo,
     const int32_t *__restrict__ atom_base, int32_t nsys, int4 *__restrict__ raw4,
-    int32_t *__restrict__ ticket, int32_t *__restrict__ cell_count,
-    int32_t *__restrict__ cell_key) {
+    int32_t *__restrict__ ticket, int32_t *__restrict__ cell_count) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nat) return;
     const SysGeom &g = geo[system_of(atom_base, nsys, a)];
@@ -166,7 +165,6 @@ __global__ __launch_bounds__(kBlock) void k_bin_dense(
         c[k] = g.pbc[k] ? wrap_idx(raw[k], g.n[k]) : clamp_idx(raw[k], g.n[k]);
     const int cell = g.cell_base + c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
     ticket[a] = atomicAdd(&cell_count[cell], 1);
-    atomicMin(&cell_key[cell], a);
     raw4[a] = make_int4(raw[0], raw[1], raw[2], cell);
 }
 
@@ -176,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_dense(
 __global__ __launch_bounds__(kBlock) void k_bin_hashed(
     const double *__restrict__ r, int32_t nat, const SysGeom *__restrict__ geo,
     int4 *__restrict__ raw4, int32_t *__restrict__ ticket, int64_t *__restrict__ hkey,
-    int64_t hmask, int32_t *__restrict__ hcount, int32_t *__restrict__ cell_key) {
+    int64_t hmask, int32_t *__restrict__ hcount) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nat) return;
     const SysGeom &g = geo[0];
@@ -199,36 +197,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_hashed(
         h = (h + 1) & hmask;
     }
     ticket[a] = atomicAdd(&hcount[h], 1);
-    atomicMin(&cell_key[h], a);
     raw4[a] = make_int4(raw[0], raw[1], raw[2], static_cast<int>(h));
-}
-
-/* Memory order of the cells: by their smallest atom index, so that the
-   cell-sorted records follow the order of the input (cells neighbouring in
-   the input are neighbours in memory). The traversal does not depend on it:
-   a cell is found by its coordinates, and inside a cell atoms stay in
-   ascending index order. w[a] = atoms of a's cell if a is the cell's
-   smallest atom, else 0; its exclusive scan at a cell's smallest atom is the
-   cell's first slot. */
-__global__ __launch_bounds__(kBlock) void k_cell_weight(const int4 *__restrict__ raw4,
-                                                        int32_t nat,
-                                                        const int32_t *__restrict__ cell_key,
-                                                        const int32_t *__restrict__ cell_count,
-                                                        int32_t *__restrict__ w) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nat) return;
-    const int c = raw4[a].w;
-    w[a] = cell_key[c] == a ? cell_count[c] : 0;
-}
-
-__global__ __launch_bounds__(kBlock) void k_cell_first(const int32_t *__restrict__ cell_key,
-                                                       const int32_t *__restrict__ cell_count,
-                                                       int64_t ncells,
-                                                       const int32_t *__restrict__ wscan,
-                                                       int32_t *__restrict__ cell_first) {
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    cell_first[c] = cell_count[c] > 0 ? wscan[cell_key[c]] : 0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(const int4 *__restrict__ raw4,
@@ -1053,37 +1022,19 @@ hipError_t launch_fill_i64(int64_t *p, int64_t n, int64_t v, hipStream_t stream)
 
 hipError_t launch_bin_dense(const double *r, int32_t nat, const SysGeom *geo,
                             const int32_t *atom_base, int32_t nsys, int4 *raw4,
-                            int32_t *ticket, int32_t *cell_count, int32_t *cell_key,
-                            hipStream_t stream) {
+                            int32_t *ticket, int32_t *cell_count, hipStream_t stream) {
     if (nat <= 0) return hipSuccess;
     k_bin_dense<<<grid_for(nat), kBlock, 0, stream>>>(r, nat, geo, atom_base, nsys, raw4,
-                                                      ticket, cell_count, cell_key);
+                                                      ticket, cell_count);
     return hipGetLastError();
 }
 
 hipError_t launch_bin_hashed(const double *r, int32_t nat, const SysGeom *geo, int4 *raw4,
                              int32_t *ticket, int64_t *hkey, int64_t hmask,
-                             int32_t *hcount, int32_t *cell_key, hipStream_t stream) {
+                             int32_t *hcount, hipStream_t stream) {
     if (nat <= 0) return hipSuccess;
     k_bin_hashed<<<grid_for(nat), kBlock, 0, stream>>>(r, nat, geo, raw4, ticket, hkey,
-                                                       hmask, hcount, cell_key);
-    return hipGetLastError();
-}
-
-hipError_t launch_cell_order(const int4 *raw4, int32_t nat, const int32_t *cell_key,
-                             const int32_t *cell_count, int64_t ncells, int32_t *w,
-                             int32_t *wscan, int64_t *partial, int32_t *cell_first,
-                             hipStream_t stream) {
-    if (nat <= 0) {
-        if (ncells > 0)
-            k_fill_i32<<<grid_for(ncells), kBlock, 0, stream>>>(cell_first, ncells, 0);
-        return hipGetLastError();
-    }
-    k_cell_weight<<<grid_for(nat), kBlock, 0, stream>>>(raw4, nat, cell_key, cell_count, w);
-    hipError_t e = launch_scan_i32_to_i32(w, nat, wscan, partial, nullptr, stream);
-    if (e != hipSuccess) return e;
-    k_cell_first<<<grid_for(ncells), kBlock, 0, stream>>>(cell_key, cell_count, ncells, wscan,
-                                                          cell_first);
+                                                       hmask, hcount);
     return hipGetLastError();
 }
 
