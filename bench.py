@@ -216,6 +216,161 @@ def batched_leg(lib, _lib, torch, dev, steps, warmup):
     }
 
 
+def _plan_step_fn(lib, _lib, torch, dev, w, f32=False, flags="ijdDS"):
+    """A steady-state build of one workload: count (async) + fill into
+    pre-allocated outputs, as bench steps run it. Returns (step, npairs)."""
+    import ctypes
+
+    import numpy as np
+
+    from matscipy_neighbours_amd.neighbours import _parse_cutoff
+    P_ = _lib.ptr
+    N = len(w["positions"])
+    cell = np.ascontiguousarray(w["cell"], dtype=np.float64)
+    inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+    pbc = np.ascontiguousarray(np.asarray(w["pbc"], bool), dtype=np.int32)
+    co = np.zeros(3)
+    rc, per_atom, pt, nc = _parse_cutoff(w["cutoff"], N)
+    types = None
+    if pt is not None:
+        types = torch.as_tensor(np.asarray(w["numbers"], np.int32), device=dev)
+    pos = torch.from_numpy(np.ascontiguousarray(w["positions"])).to(dev)
+    plan = _lib.Plan(0)
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    n = ctypes.c_int64(0)
+    _lib.check(lib.mnl_plan_count(plan.handle, sp, P_(co), P_(cell), P_(inv), P_(pbc), N,
+                                  P_(pos), rc, None, P_(pt), nc, P_(types), -1, ctypes.byref(n)))
+    P = int(n.value)
+    fdt = torch.float32 if f32 else torch.float64
+    out = dict(i=torch.empty(P, dtype=torch.int32, device=dev),
+               j=torch.empty(P, dtype=torch.int32, device=dev),
+               D=torch.empty((P, 3), dtype=fdt, device=dev),
+               d=torch.empty(P, dtype=fdt, device=dev),
+               S=torch.empty((P, 3), dtype=torch.int32, device=dev))
+    np_dev = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    def step():
+        _lib.check(lib.mnl_plan_count_async(plan.handle, sp, P_(co), P_(cell), P_(inv), P_(pbc),
+                                            N, P_(pos), rc, None, P_(pt), nc, P_(types), -1,
+                                            P_(np_dev)))
+        _lib.check(lib.mnl_plan_fill_async(plan.handle, sp, P, *(P_(out[q]) if q in flags else
+                                                                None for q in "ijDdS"),
+                                           1 if f32 else 0))
+    step.plan, step.out, step.sp, step.N = plan, out, sp, N
+    return step, P
+
+
+def _time_steps(torch, stream, step, steps, warmup):
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def other_workloads(lib, _lib, torch, dev, steps, warmup, with_cpu):
+    """The smaller workloads: the 1000-atom latency case, the per-type slab,
+    and the differentiable bcc case (forward f64 / f32 and backward)."""
+    import ctypes
+
+    import numpy as np
+
+    from matscipy_neighbours_amd import workloads
+    stream = torch.cuda.current_stream(dev)
+    res = {}
+
+    # 1000 atoms, 10 A box, rc 2.5: latency of one full build (count, host
+    # read of the pair total, fill) on the host clock
+    w = workloads.uniform_box()
+    step, P = _plan_step_fn(lib, _lib, torch, dev, w)
+    cell = np.ascontiguousarray(w["cell"])
+    inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+    pbc = np.ones(3, np.int32)
+    pos = torch.from_numpy(w["positions"]).to(dev)
+    o = step.out
+    n = ctypes.c_int64(0)
+
+    def build():
+        _lib.check(lib.mnl_plan_count(step.plan.handle, step.sp, _lib.ptr(np.zeros(3)),
+                                      _lib.ptr(cell), _lib.ptr(inv), _lib.ptr(pbc), 1000,
+                                      _lib.ptr(pos), 2.5, None, None, 0, None, -1,
+                                      ctypes.byref(n)))
+        _lib.check(lib.mnl_plan_fill(step.plan.handle, step.sp, *(_lib.ptr(o[q])
+                                                                  for q in "ijDdS")))
+        torch.cuda.synchronize()
+    for _ in range(20):
+        build()
+    t = []
+    for _ in range(200):
+        t0 = time.perf_counter()
+        build()
+        t.append(time.perf_counter() - t0)
+    entry = {"atoms": 1000, "pairs": P, "us_per_build": float(np.median(t)) * 1e6,
+             "how": "count (waits for the pair total) + fill, host clock, median of 200"}
+    if with_cpu:
+        import oracle
+        if oracle.ref_available():
+            secs, npairs, threads = oracle.ref_time("ijdDS", np.zeros(3), cell, inv,
+                                                    w["pbc"], w["positions"], 2.5, 51)
+            entry["cpu_reference_us"] = float(np.median(secs[1:])) * 1e6
+            entry["cpu_threads"] = int(threads)
+    res["uniform_1000_rc2.5"] = entry
+
+    # the slab: 200,000 atoms, two species, per-type cutoffs, pbc TTF
+    w = workloads.triclinic_slab()
+    step, P = _plan_step_fn(lib, _lib, torch, dev, w)
+    ms = _time_steps(torch, stream, step, steps, warmup)
+    N = step.N
+    b = 24 * N + 4 * N + 52 * P
+    res["slab_200k_pertype"] = {
+        "atoms": N, "pairs": P, "ms_per_step": ms, "pairs_per_s": P / (ms * 1e-3),
+        "hbm_frac": b / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "backend": "dense" if lib.mnl_plan_backend(step.plan.handle) == 0 else "hashed"}
+
+    # differentiable bcc: forward ijdD (f64, f32) and the backward kernel
+    w = workloads.bcc_sheared()
+    fwd = {}
+    for f32 in (False, True):
+        step, P = _plan_step_fn(lib, _lib, torch, dev, w, f32=f32, flags="ijDd")
+        fwd["f32" if f32 else "f64"] = (_time_steps(torch, stream, step, steps, warmup), step)
+    step = fwd["f64"][1]
+    o = step.out
+    step()
+    N = step.N
+    gD = torch.randn((P, 3), dtype=torch.float64, device=dev)
+    gd = torch.randn(P, dtype=torch.float64, device=dev)
+    gpos = torch.empty((N, 3), dtype=torch.float64, device=dev)
+    gcell = torch.empty((3, 3), dtype=torch.float64, device=dev)
+    # S is needed for the cell gradient: one more fill with S
+    S = torch.empty((P, 3), dtype=torch.int32, device=dev)
+    _lib.check(lib.mnl_plan_fill_async(step.plan.handle, step.sp, P, None, None, None, None,
+                                       _lib.ptr(S), 0))
+
+    def bwd():
+        _lib.check(lib.mnl_plan_pair_backward(step.plan.handle, step.sp, P, N,
+                                              _lib.ptr(o["i"]), _lib.ptr(o["j"]), _lib.ptr(S),
+                                              _lib.ptr(o["D"]), _lib.ptr(o["d"]), _lib.ptr(gD),
+                                              _lib.ptr(gd), 0, _lib.ptr(gpos), _lib.ptr(gcell)))
+    ms_b = _time_steps(torch, stream, bwd, steps, warmup)
+    res["bcc_131k_autograd"] = {
+        "atoms": N, "pairs": P,
+        "forward_ijdD_f64_ms": fwd["f64"][0], "forward_ijdD_f32_ms": fwd["f32"][0],
+        "backward_ms": ms_b,
+        "forward_f64_hbm_frac": (24 * N + 40 * P) / (fwd["f64"][0] * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "forward_f32_hbm_frac": (24 * N + 24 * P) / (fwd["f32"][0] * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "backward_hbm_frac": (108 * P + 24 * N) / (ms_b * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "how": "forward = count + fill of i, j, D, d; backward reads i, j, S, D, d, dL/dD, "
+               "dL/dd (108 B/pair) and writes dL/dr (24 B/atom) and dL/dcell"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +379,8 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-systems leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other workloads (1000-atom latency, slab, bcc autograd)")
     ap.add_argument("--reps", type=int, default=64, help="fcc unit cells per edge")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (to scale HBM "
@@ -357,6 +514,13 @@ def main():
         batched = batched_leg(lib, _lib, torch, dev, args.steps, args.warmup)
         _log(f"batched: {batched}")
 
+    extra = None
+    if not args.no_extra:
+        _log("other workloads")
+        extra = other_workloads(lib, _lib, torch, dev, args.steps, args.warmup,
+                                not args.no_cpu)
+        _log(f"other workloads: {extra}")
+
     cpu = None
     if not args.no_cpu:
         _log("timing the CPU baseline")
@@ -396,6 +560,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "batched": batched,
+        "other_workloads": extra,
     }
     print(json.dumps(line), flush=True)
 

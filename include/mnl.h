@@ -46,9 +46,9 @@ enum {
     MNL_Q_SHIFT = 16   /* S */
 };
 
-/* Cell-grid backend: the reference rule (cell_list.cc:193-196) picks hashed
-   when ncells > max(2^20, 8*nat); the engine additionally picks hashed for
-   grids that are mostly empty. The backend never changes the output. */
+/* Cell-grid backend: AUTO applies the reference rule (cell_list.cc:193-196),
+   hashed when ncells > max(2^20, 8*nat); DENSE and HASHED force one. The
+   backend never changes the output. */
 enum { MNL_BACKEND_AUTO = -1, MNL_BACKEND_DENSE = 0, MNL_BACKEND_HASHED = 1 };
 
 /* Message of the last error on this thread ("" if none). */
@@ -187,7 +187,8 @@ int mnl_plan_fill_f32(mnl_plan *plan, void *stream, int32_t *first,
  * memory once the workspaces are large enough (the first build of a given
  * size, outside capture, sizes them; growing under capture is an error).
  * The pair count is written to *npairs_device ([device] int64, may be NULL).
- * Per-type cutoff tables are not capturable.
+ * A per-type cutoff table is uploaded (waiting for the stream) only when it
+ * differs from the previous one, so only its first build is not capturable.
  */
 int mnl_plan_count_async(mnl_plan *plan, void *stream, const double origin[3],
                          const double cell[9], const double inv_cell[9],

@@ -135,6 +135,8 @@ struct mnl_plan {
     int64_t npairs = 0;
     int32_t ncut = 0;
     hipStream_t stream = nullptr;
+    /* host copy of the per-type table on the device */
+    std::vector<double> pt_host;
     /* batched geometry uploaded by mnl_plan_set_batch and still in place */
     bool batch_ready = false;
 };
@@ -209,6 +211,7 @@ int ensure_workspaces(mnl_plan *p, hipStream_t stream, int32_t nat, int64_t cell
                     "uncaptured build of this size first");
     MNL_HIP(hipStreamSynchronize(stream));
     for (const Need &nd : needs) MNL_HIP(nd.b->ensure(nd.bytes));
+    p->pt_host.clear(); /* buffers may have moved: upload the table again */
     return MNL_SUCCESS;
 }
 
@@ -315,13 +318,21 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
                               p->cell_base.as<int32_t>(), nat,
                               static_cast<int32_t>(hashed ? 0 : ncells), stream));
     if (p->cut == CUT_PER_TYPE && ncutoffs > 0) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        MNL_HIP(hipStreamIsCapturing(stream, &cs));
-        if (cs != hipStreamCaptureStatusNone)
-            return fail("per-type cutoffs cannot be used under graph capture");
-        const size_t b = sizeof(double) * size_t(ncutoffs) * size_t(ncutoffs);
-        MNL_HIP(hipMemcpyAsync(p->pt.ptr, per_type_sq, b, hipMemcpyHostToDevice, stream));
-        MNL_HIP(hipStreamSynchronize(stream));
+        /* the table is uploaded only when it differs from the plan's copy, so
+           steady-state builds with fixed cutoffs stay asynchronous */
+        const size_t nt = size_t(ncutoffs) * size_t(ncutoffs);
+        if (p->pt_host.size() != nt ||
+            std::memcmp(p->pt_host.data(), per_type_sq, nt * sizeof(double)) != 0) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            MNL_HIP(hipStreamIsCapturing(stream, &cs));
+            if (cs != hipStreamCaptureStatusNone)
+                return fail("a new per-type cutoff table cannot be uploaded under graph "
+                            "capture: run one uncaptured build with it first");
+            p->pt_host.assign(per_type_sq, per_type_sq + nt);
+            MNL_HIP(hipMemcpyAsync(p->pt.ptr, p->pt_host.data(), nt * sizeof(double),
+                                   hipMemcpyHostToDevice, stream));
+            MNL_HIP(hipStreamSynchronize(stream));
+        }
     }
     return enqueue_build(p, stream, positions, per_atom, types);
 }

@@ -109,14 +109,19 @@ __device__ __forceinline__ int mbcnt64(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-/* XCD-aware remap: the hardware deals workgroups round-robin to the 8 XCDs;
-   hand each XCD a contiguous range of logical blocks so that neighbouring
-   cells share one L2. A bijection for any grid size. */
+/* XCD-aware remap: the hardware deals workgroups round-robin to the 8 XCDs.
+   Hand each XCD runs of kXcdRun consecutive logical blocks (64 cells), runs
+   dealt round-robin: neighbouring cells share an L2, and work that is
+   concentrated in one part of the grid (a slab under vacuum) still spreads
+   over all XCDs. A bijection for any grid size (the tail that does not fill
+   a round of runs keeps the hardware order). */
 __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
     constexpr int kXcd = 8;
-    const int q = nblocks / kXcd, r = nblocks % kXcd;
+    constexpr int kXcdRun = 16;
+    const int full = nblocks / (kXcd * kXcdRun) * (kXcd * kXcdRun);
+    if (b >= full) return b;
     const int x = b % kXcd, k = b / kXcd;
-    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+    return ((k / kXcdRun) * kXcd + x) * kXcdRun + (k % kXcdRun);
 }
 
 /* ---------------------------------------------------------------- fills */
@@ -405,6 +410,12 @@ __device__ __forceinline__ cptr<T> as_const(const T *p) {
     return (cptr<T>)(p);
 }
 
+/* A pointer into LDS, so that reads through it are ds_read (not flat). */
+using lds_dptr = const __attribute__((address_space(3))) double *;
+
+/* Kernel-side cut mode: per-type cutoffs with the table staged in LDS. */
+constexpr int CUT_PER_TYPE_LDS = 3;
+
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 /* Element i of a read-only record array through the constant address space
@@ -597,7 +608,7 @@ __device__ __forceinline__ void load_candidate(const PairArgs &A, const HomeCell
     C.py = pj.y;
     C.pz = pj.z;
     C.pw = pj.w;
-    C.tj = CUT == CUT_PER_TYPE ? A.stype[sj] : 0;
+    C.tj = (CUT == CUT_PER_TYPE || CUT == CUT_PER_TYPE_LDS) ? A.stype[sj] : 0;
     if (FILL) {
         const int4 bj = A.sinfo[sj];
         C.j = bj.w;
@@ -614,7 +625,7 @@ __device__ __forceinline__ void load_candidate(const PairArgs &A, const HomeCell
 template <int CUT>
 __device__ __forceinline__ bool pair_test(const PairArgs &A, const HomeCell &H, const Cand &C,
                                           int si, const double4 &pi, double &d0, double &d1,
-                                          double &d2) {
+                                          double &d2, lds_dptr pt_lds) {
     d0 = (C.px - pi.x) + C.o0;
     d1 = (C.py - pi.y) + C.o1;
     d2 = (C.pz - pi.z) + C.o2;
@@ -623,11 +634,12 @@ __device__ __forceinline__ bool pair_test(const PairArgs &A, const HomeCell &H, 
     if (CUT == CUT_PER_ATOM) {
         const double cc = pi.w + C.pw;
         ok = ok & (r2 < cc * cc);
-    } else if (CUT == CUT_PER_TYPE) {
+    } else if (CUT == CUT_PER_TYPE || CUT == CUT_PER_TYPE_LDS) {
         const int ti = as_const(A.stype)[si];
         if (ti >= 0 && ti < A.ncut) {
             const bool tj_in = (C.tj >= 0) & (C.tj < A.ncut);
-            const double lim = A.pt_sq[ti * A.ncut + (tj_in ? C.tj : 0)];
+            const int e = ti * A.ncut + (tj_in ? C.tj : 0);
+            const double lim = CUT == CUT_PER_TYPE_LDS ? pt_lds[e] : A.pt_sq[e];
             ok = ok & (!tj_in | (r2 < lim));
         }
     }
@@ -646,7 +658,8 @@ __device__ __forceinline__ bool pair_test(const PairArgs &A, const HomeCell &H, 
  * count. Home atoms are taken in groups of 64.
  */
 template <int CUT, bool HASHED>
-__device__ __forceinline__ void count_cell(const PairArgs &A, int h, SegLds &S) {
+__device__ __forceinline__ void count_cell(const PairArgs &A, int h, SegLds &S,
+                                           lds_dptr pt_sq) {
     HomeCell H;
     if (!load_home<HASHED>(A, h, H)) return;
     const int lane = lane_id();
@@ -667,7 +680,7 @@ __device__ __forceinline__ void count_cell(const PairArgs &A, int h, SegLds &S) 
                     const int si = H.hb + grp0 + t;
                     const double4 pi = load_const(A.spos, si);
                     double d0, d1, d2;
-                    const bool ok = pair_test<CUT>(A, H, C, si, pi, d0, d1, d2) &
+                    const bool ok = pair_test<CUT>(A, H, C, si, pi, d0, d1, d2, pt_sq) &
                                     (c != self_base + t);
                     const uint64_t mask = __builtin_amdgcn_ballot_w64(ok);
                     mlo = write_lane(mlo, static_cast<uint32_t>(mask), t);
@@ -681,14 +694,30 @@ __device__ __forceinline__ void count_cell(const PairArgs &A, int h, SegLds &S) 
     }
 }
 
+/* Per-type cutoff tables up to this many entries are staged in LDS once per
+   block (the test reads one entry per candidate and home atom); larger ones
+   are read from global memory. */
+constexpr int kLdsTypes = 256;
+
+template <int CUT>
+__device__ __forceinline__ lds_dptr stage_types(const PairArgs &A, double *s_pt) {
+    if (CUT != CUT_PER_TYPE_LDS) return nullptr;
+    const int nt = A.ncut * A.ncut;
+    for (int k = threadIdx.x; k < nt; k += kBlock) s_pt[k] = A.pt_sq[k];
+    __syncthreads();
+    return (lds_dptr)(s_pt);
+}
+
 template <int CUT, bool HASHED>
 __global__ __launch_bounds__(kBlock) void k_count(PairArgs A) {
     __shared__ SegLds s_lds[kWavesPerBlock];
+    __shared__ double s_pt[CUT == CUT_PER_TYPE_LDS ? kLdsTypes : 1];
+    const lds_dptr pt_sq = stage_types<CUT>(A, s_pt);
     const int wib = uniform(threadIdx.x / kWave);
     const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
     const int h = blk * kWavesPerBlock + wib;
     if (h >= A.home_cells) return;
-    count_cell<CUT, HASHED>(A, h, s_lds[wib]);
+    count_cell<CUT, HASHED>(A, h, s_lds[wib], pt_sq);
 }
 
 /* ------------------------------------------------------------- fill pass */
@@ -783,7 +812,8 @@ __device__ __forceinline__ void flush_ring(const PairArgs &A, const HomeCell &H,
  * ranks continue across windows, so an atom's row fills front to back.
  */
 template <int MODE, int CUT, bool HASHED>
-__device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L) {
+__device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L,
+                                          lds_dptr pt_sq) {
     HomeCell H;
     if (!load_home<HASHED>(A, h, H)) return;
     const int lane = lane_id();
@@ -821,8 +851,9 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L) 
                         const int cb = w0 + k * kWave;
                         if (cb >= total) break;
                         double d0, d1, d2;
-                        const bool ok = pair_test<CUT>(A, H, C[k], si, pi, d0, d1, d2) &
-                                        (cb + lane != self_base + t);
+                        const bool ok =
+                            pair_test<CUT>(A, H, C[k], si, pi, d0, d1, d2, pt_sq) &
+                            (cb + lane != self_base + t);
                         const uint64_t mask = __builtin_amdgcn_ballot_w64(ok);
                         if (mask) {
                             const int r = mbcnt64(mask);
@@ -861,11 +892,13 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L) 
 template <int MODE, int CUT, bool HASHED>
 __global__ __launch_bounds__(kBlock) void k_fill(PairArgs A) {
     __shared__ FillLds s_lds[kWavesPerBlock];
+    __shared__ double s_pt[CUT == CUT_PER_TYPE_LDS ? kLdsTypes : 1];
+    const lds_dptr pt_sq = stage_types<CUT>(A, s_pt);
     const int wib = uniform(threadIdx.x / kWave);
     const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
     const int h = blk * kWavesPerBlock + wib;
     if (h >= A.home_cells) return;
-    fill_cell<MODE, CUT, HASHED>(A, h, s_lds[wib]);
+    fill_cell<MODE, CUT, HASHED>(A, h, s_lds[wib], pt_sq);
 }
 
 /* Device-side parameter set-up (one thread): keeps the whole build free of
@@ -1133,6 +1166,8 @@ hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
     a.xcd_blocks = static_cast<int32_t>(nb);
     if (cut == CUT_PER_ATOM)
         launch_pairs_h<CUT_PER_ATOM>(mode, hashed, a, nb, stream);
+    else if (cut == CUT_PER_TYPE && a.ncut * a.ncut <= kLdsTypes)
+        launch_pairs_h<CUT_PER_TYPE_LDS>(mode, hashed, a, nb, stream);
     else if (cut == CUT_PER_TYPE)
         launch_pairs_h<CUT_PER_TYPE>(mode, hashed, a, nb, stream);
     else
