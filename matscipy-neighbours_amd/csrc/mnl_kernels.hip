@@ -1010,14 +1010,23 @@ __global__ __launch_bounds__(kBlock) void k_pair_backward(
     }
 }
 
-/* Sum of nb per-block cell partials in a fixed order (one wave). */
-__global__ void k_cell_reduce(const double *__restrict__ partial, int nb,
-                              double *__restrict__ grad_cell) {
-    const int k = threadIdx.x;
-    if (k >= 9) return;
-    double t = 0.0;
-    for (int b = 0; b < nb; b++) t += partial[static_cast<int64_t>(b) * 9 + k];
-    grad_cell[k] = t;
+/* Sum of nb per-block cell partials in a fixed order (one block): each
+   thread adds a strided subset, then a fixed-shape tree over the block. */
+__global__ __launch_bounds__(kBlock) void k_cell_reduce(const double *__restrict__ partial,
+                                                        int nb, double *__restrict__ grad_cell) {
+    __shared__ double s_sum[kBlock];
+    for (int k = 0; k < 9; k++) {
+        double t = 0.0;
+        for (int b = threadIdx.x; b < nb; b += kBlock) t += partial[static_cast<int64_t>(b) * 9 + k];
+        s_sum[threadIdx.x] = t;
+        __syncthreads();
+        for (int w = kBlock / 2; w > 0; w >>= 1) {
+            if (static_cast<int>(threadIdx.x) < w) s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) grad_cell[k] = s_sum[0];
+        __syncthreads();
+    }
 }
 
 __global__ void k_fill_f64(double *p, int64_t n, double v) {
@@ -1215,7 +1224,7 @@ hipError_t launch_pair_backward(int64_t npairs, int32_t nat, const int32_t *firs
                 npairs, first, secnd, shift, static_cast<const double *>(D),
                 static_cast<const double *>(d), static_cast<const double *>(gD),
                 static_cast<const double *>(gd), grad_pos, cell_partial);
-        k_cell_reduce<<<1, kWave, 0, stream>>>(cell_partial, nb, grad_cell);
+        k_cell_reduce<<<1, kBlock, 0, stream>>>(cell_partial, nb, grad_cell);
     } else {
         k_fill_f64<<<1, kWave, 0, stream>>>(grad_cell, 9, 0.0);
     }
