@@ -218,6 +218,34 @@ const int64_t *mnl_plan_offsets(const mnl_plan *plan);
 /* Backend chosen by the last count (MNL_BACKEND_DENSE or _HASHED). */
 int mnl_plan_backend(const mnl_plan *plan);
 
+/*
+ * Dense fixed-capacity neighbour matrix (the reference's neighbour_matrix,
+ * src/libneighbours/neighbour_list.cc:206-254, and its device form
+ * neighbour_list_gpu.cc:523-570): row i holds atom i's neighbours in the
+ * pair list's order, idx [nat*K] int32 = j, dist [nat*K*3] float64 = D,
+ * unused slots 0; count [nat] int32 = true neighbour count (rows keep only
+ * the first K). Same geometry arguments as mnl_plan_count. One traversal,
+ * no count pass. All outputs [device], caller-owned.
+ * mnl_plan_matrix waits and sets *overflow [host] to 1 if some atom has
+ * more than K neighbours (the reference raises then); the async form writes
+ * that flag to *overflow_device ([device] int32, may be NULL).
+ */
+int mnl_plan_matrix(mnl_plan *plan, void *stream, const double origin[3],
+                    const double cell[9], const double inv_cell[9], const int32_t pbc[3],
+                    int32_t nat, const double *positions, double cutoff,
+                    const double *per_atom_cutoff, const double *per_type_cutoff_sq,
+                    int32_t ncutoffs, const int32_t *types, int backend,
+                    int32_t max_neighbours, int32_t *idx, double *dist, int32_t *count,
+                    int32_t *overflow);
+int mnl_plan_matrix_async(mnl_plan *plan, void *stream, const double origin[3],
+                          const double cell[9], const double inv_cell[9],
+                          const int32_t pbc[3], int32_t nat, const double *positions,
+                          double cutoff, const double *per_atom_cutoff,
+                          const double *per_type_cutoff_sq, int32_t ncutoffs,
+                          const int32_t *types, int backend, int32_t max_neighbours,
+                          int32_t *idx, double *dist, int32_t *count,
+                          int32_t *overflow_device);
+
 /* ---------------------------------------------------------------- batches
  *
  * Many independent systems in one build: atoms concatenated in system

@@ -216,13 +216,12 @@ int ensure_workspaces(mnl_plan *p, hipStream_t stream, int32_t nat, int64_t cell
 }
 
 /*
- * The launch sequence of a build whose geometry (geo, atom_base, cell_base)
- * is already on the device: clear -> bin -> scan -> scatter -> rank/gather
- * -> count -> scan. Leaves the pair total in p->total (device); does not
- * synchronise and records no memset, so it can be captured in a graph.
+ * Binning of a build whose geometry (geo, atom_base, cell_base) is already
+ * on the device: clear -> bin -> scan -> scatter -> rank/gather. Records no
+ * memset and does not synchronise (capturable).
  */
-int enqueue_build(mnl_plan *p, hipStream_t stream, const double *positions,
-                  const double *per_atom, const int32_t *types) {
+int enqueue_binning(mnl_plan *p, hipStream_t stream, const double *positions,
+                    const double *per_atom, const int32_t *types) {
     const bool hashed = p->backend == MNL_BACKEND_HASHED;
     const int64_t cells = p->home_cells;
     const int32_t nat = p->nat;
@@ -249,6 +248,19 @@ int enqueue_build(mnl_plan *p, hipStream_t stream, const double *positions,
                                per_atom, types, p->spos.as<double4>(), p->sinfo.as<int4>(),
                                p->cut == CUT_PER_TYPE ? p->stype.as<int32_t>() : nullptr,
                                stream));
+    return MNL_SUCCESS;
+}
+
+/*
+ * binning -> count -> scan. Leaves the pair total in p->total (device);
+ * does not synchronise and records no memset, so it can be captured in a
+ * graph.
+ */
+int enqueue_build(mnl_plan *p, hipStream_t stream, const double *positions,
+                  const double *per_atom, const int32_t *types) {
+    if (enqueue_binning(p, stream, positions, per_atom, types) != MNL_SUCCESS) return MNL_ERROR;
+    const bool hashed = p->backend == MNL_BACKEND_HASHED;
+    const int32_t nat = p->nat;
     PairArgs a = pair_args(p);
     MNL_HIP(launch_pairs(0, p->cut, hashed, a, stream));
     MNL_HIP(launch_scan_i32_to_i64(p->counts.as<int32_t>(), nat, p->offsets.as<int64_t>(),
@@ -264,7 +276,7 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
                   const double cell[9], const double inv_cell[9], const int32_t pbc[3],
                   int32_t nat, const double *positions, double cutoff,
                   const double *per_atom, const double *per_type_sq, int32_t ncutoffs,
-                  const int32_t *types, int backend) {
+                  const int32_t *types, int backend, bool with_count = true) {
     p->counted = false;
     p->batch_ready = false;
     if (nat < 0) return fail("negative atom count");
@@ -334,6 +346,7 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
             MNL_HIP(hipStreamSynchronize(stream));
         }
     }
+    if (!with_count) return enqueue_binning(p, stream, positions, per_atom, types);
     return enqueue_build(p, stream, positions, per_atom, types);
 }
 
@@ -769,6 +782,82 @@ void mnl_result_free(mnl_result *r) {
     if (!r) return;
     free_result_buffers(r);
     delete r;
+}
+
+namespace {
+int matrix_enqueue(mnl_plan *p, hipStream_t st, const double origin[3], const double cell[9],
+                   const double inv_cell[9], const int32_t pbc[3], int32_t nat,
+                   const double *positions, double cutoff, const double *per_atom,
+                   const double *per_type_sq, int32_t ncutoffs, const int32_t *types,
+                   int backend, int32_t max_neighbours, int32_t *idx, double *dist,
+                   int32_t *count, int32_t *overflow) {
+    if (!origin || !cell || !inv_cell || !pbc) return fail("geometry pointer is NULL");
+    if (nat > 0 && !positions) return fail("positions is NULL");
+    if (max_neighbours < 0) return fail("max_neighbours must be non-negative");
+    const int64_t slots = int64_t(nat) * max_neighbours;
+    if (nat > 0 && ((slots > 0 && (!idx || !dist)) || !count))
+        return fail("matrix output pointer is NULL");
+    if (enqueue_count(p, st, origin, cell, inv_cell, pbc, nat, positions, cutoff, per_atom,
+                      per_type_sq, ncutoffs, types, backend, false) != MNL_SUCCESS)
+        return MNL_ERROR;
+    if (overflow) MNL_HIP(launch_fill_i32(overflow, 1, 0, st));
+    if (slots > 0) {
+        MNL_HIP(launch_fill_i32(idx, slots, 0, st));
+        MNL_HIP(launch_fill_i64(reinterpret_cast<int64_t *>(dist), 3 * slots, 0, st));
+    }
+    if (nat == 0) return MNL_SUCCESS;
+    PairArgs a = pair_args(p);
+    a.counts = count;
+    a.out_j = idx;
+    a.out_D = dist;
+    a.max_nb = max_neighbours;
+    a.overflow = overflow;
+    MNL_HIP(launch_pairs(3, p->cut, p->backend == MNL_BACKEND_HASHED, a, st));
+    p->counted = false; /* the plan holds no pair offsets after a matrix build */
+    return MNL_SUCCESS;
+}
+}  // namespace
+
+int mnl_plan_matrix_async(mnl_plan *p, void *stream, const double origin[3],
+                          const double cell[9], const double inv_cell[9],
+                          const int32_t pbc[3], int32_t nat, const double *positions,
+                          double cutoff, const double *per_atom_cutoff,
+                          const double *per_type_cutoff_sq, int32_t ncutoffs,
+                          const int32_t *types, int backend, int32_t max_neighbours,
+                          int32_t *idx, double *dist, int32_t *count,
+                          int32_t *overflow_device) {
+    if (!p) return fail("plan is NULL");
+    DeviceScope ds(p->device);
+    if (!ds.ok) return fail("cannot switch to the plan's device");
+    return matrix_enqueue(p, static_cast<hipStream_t>(stream), origin, cell, inv_cell, pbc, nat,
+                          positions, cutoff, per_atom_cutoff, per_type_cutoff_sq, ncutoffs,
+                          types, backend, max_neighbours, idx, dist, count, overflow_device);
+}
+
+int mnl_plan_matrix(mnl_plan *p, void *stream, const double origin[3], const double cell[9],
+                    const double inv_cell[9], const int32_t pbc[3], int32_t nat,
+                    const double *positions, double cutoff, const double *per_atom_cutoff,
+                    const double *per_type_cutoff_sq, int32_t ncutoffs, const int32_t *types,
+                    int backend, int32_t max_neighbours, int32_t *idx, double *dist,
+                    int32_t *count, int32_t *overflow) {
+    if (!p) return fail("plan is NULL");
+    if (!overflow) return fail("overflow is NULL");
+    DeviceScope ds(p->device);
+    if (!ds.ok) return fail("cannot switch to the plan's device");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    /* the flag lives in the plan's total slot (an int64, used here as int32) */
+    if (p->total.cap < sizeof(int64_t)) MNL_HIP(p->total.ensure(sizeof(int64_t)));
+    int32_t *flag = p->total.as<int32_t>();
+    if (matrix_enqueue(p, st, origin, cell, inv_cell, pbc, nat, positions, cutoff,
+                       per_atom_cutoff, per_type_cutoff_sq, ncutoffs, types, backend,
+                       max_neighbours, idx, dist, count, flag) != MNL_SUCCESS)
+        return MNL_ERROR;
+    MNL_HIP(hipMemcpyAsync(p->host_total, flag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    MNL_HIP(hipStreamSynchronize(st));
+    int32_t v = 0;
+    std::memcpy(&v, p->host_total, sizeof v);
+    *overflow = nat > 0 ? v : 0;
+    return MNL_SUCCESS;
 }
 
 int mnl_plan_set_batch(mnl_plan *p, void *stream, int32_t nsys, const int32_t *atom_offsets,

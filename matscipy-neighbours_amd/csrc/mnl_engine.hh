@@ -77,6 +77,8 @@ struct PairArgs {
     int32_t *counts;           /* count: [N] by global atom index */
     const int64_t *offsets;    /* fill: [N+1] */
     int64_t capacity;          /* fill: pairs beyond this are not written */
+    int32_t max_nb;            /* matrix fill: slots per row */
+    int32_t *overflow;         /* matrix fill: set to 1 if a row exceeds max_nb */
     int32_t *out_i;
     int32_t *out_j;
     void *out_D;
@@ -115,7 +117,8 @@ hipError_t launch_scan_i32_to_i64(const int32_t *in, int64_t n, int64_t *out,
                                   int64_t *partial, int64_t *total,
                                   hipStream_t stream);
 /* Pair kernels (one wave per home cell). mode 0 = count, 1 = fill f64
-   outputs, 2 = fill f32 outputs. */
+   outputs, 2 = fill f32 outputs, 3 = dense matrix (rows of max_nb slots:
+   out_j = neighbour index, out_D = D, counts = row counts; no count pass). */
 hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
                         hipStream_t stream);
 hipError_t launch_set_params(SysGeom *geo, const SysGeom &g, int32_t *atom_base,

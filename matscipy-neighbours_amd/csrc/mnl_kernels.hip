@@ -358,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const TI *__restrict__ in
 
 /* ---------------------------------------------------------------- pairs */
 
-enum { MODE_COUNT = 0, MODE_FILL64 = 1, MODE_FILL32 = 2 };
+enum { MODE_COUNT = 0, MODE_FILL64 = 1, MODE_FILL32 = 2, MODE_MATRIX = 3 };
 
 /* Pack three small signed integers (|v| < 512) into 30 bits and back. */
 __device__ __forceinline__ int pack3(int a, int b, int c) {
@@ -792,11 +792,23 @@ __device__ __forceinline__ void flush_ring(const PairArgs &A, const HomeCell &H,
         const int e = (tail + lane) & (kRing - 1);
         const int tk = L.rtk[e];
         const int t = tk & (kWave - 1);
-        const int64_t pos = L.hpos[t] + (tk >> 6);
-        write_pair<MODE>(A, pos, L.hi[t], L.rj[e], L.rdx[e], L.rdy[e], L.rdz[e],
-                         shift_of(H.p0, L.hbx[t], L.rc0[e], H.rn0),
-                         shift_of(H.p1, L.hby[t], L.rc1[e], H.rn1),
-                         shift_of(H.p2, L.hbz[t], L.rc2[e], H.rn2));
+        const int rank = tk >> 6;
+        const int64_t pos = L.hpos[t] + rank;
+        if (MODE == MODE_MATRIX) {
+            /* row of the dense matrix: neighbour index and D, ranks < K */
+            if (rank < A.max_nb) {
+                A.out_j[pos] = L.rj[e];
+                double *D = static_cast<double *>(A.out_D) + 3 * pos;
+                D[0] = L.rdx[e];
+                D[1] = L.rdy[e];
+                D[2] = L.rdz[e];
+            }
+        } else {
+            write_pair<MODE>(A, pos, L.hi[t], L.rj[e], L.rdx[e], L.rdy[e], L.rdz[e],
+                             shift_of(H.p0, L.hbx[t], L.rc0[e], H.rn0),
+                             shift_of(H.p1, L.hby[t], L.rc1[e], H.rn1),
+                             shift_of(H.p2, L.hbz[t], L.rc2[e], H.rn2));
+        }
     }
     wave_sync_lds();
 }
@@ -821,7 +833,10 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L,
         const int mg = uniform(min(kWave, H.m - grp0));
         if (lane < mg) {
             const int4 inf = A.sinfo[H.hb + grp0 + lane];
-            L.hpos[lane] = A.offsets[H.abase + inf.w];
+            /* pair list: the atom's row offset; matrix: its row of K slots */
+            L.hpos[lane] = MODE == MODE_MATRIX
+                               ? static_cast<int64_t>(H.abase + inf.w) * A.max_nb
+                               : A.offsets[H.abase + inf.w];
             L.hbx[lane] = inf.x;
             L.hby[lane] = inf.y;
             L.hbz[lane] = inf.z;
@@ -886,6 +901,12 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L,
             __builtin_amdgcn_wave_barrier();
         }
         if (head > tail) flush_ring<MODE>(A, H, L, tail, head - tail);
+        /* the matrix has no count pass: the traversal's per-atom total is
+           the row count */
+        if (MODE == MODE_MATRIX && lane < mg) {
+            A.counts[H.abase + L.hi[lane]] = acc;
+            if (acc > A.max_nb && A.overflow) atomicMax(A.overflow, 1);
+        }
     }
 }
 
@@ -1157,11 +1178,16 @@ static void launch_pairs_h(int mode, bool hashed, const PairArgs &a, unsigned nb
             launch_fill_t<MODE_FILL64, CUT, true>(a, nb, s);
         else
             launch_fill_t<MODE_FILL64, CUT, false>(a, nb, s);
-    } else {
+    } else if (mode == MODE_FILL32) {
         if (hashed)
             launch_fill_t<MODE_FILL32, CUT, true>(a, nb, s);
         else
             launch_fill_t<MODE_FILL32, CUT, false>(a, nb, s);
+    } else {
+        if (hashed)
+            launch_fill_t<MODE_MATRIX, CUT, true>(a, nb, s);
+        else
+            launch_fill_t<MODE_MATRIX, CUT, false>(a, nb, s);
     }
 }
 

@@ -15,6 +15,7 @@ from .neighbours import (
     mic,
     neighbour_list,
     neighbour_list_batch,
+    neighbour_matrix,
     triplet_list,
 )
 
@@ -22,6 +23,7 @@ __all__ = [
     "neighbour_list",
     "neighbour_list_batch",
     "neighbour_list_autograd",
+    "neighbour_matrix",
     "first_neighbours",
     "get_jump_indicies",
     "triplet_list",

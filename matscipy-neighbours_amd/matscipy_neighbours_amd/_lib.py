@@ -56,6 +56,13 @@ SIGNATURES = [
     ("mnl_plan_counts", ctypes.c_int, [_vp, _vp, _vp]),
     ("mnl_plan_offsets", _vp, [_vp]),
     ("mnl_plan_backend", ctypes.c_int, [_vp]),
+    ("mnl_plan_matrix", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp, ctypes.c_double, _vp, _vp,
+      ctypes.c_int32, _vp, ctypes.c_int, ctypes.c_int32, _vp, _vp, _vp,
+      ctypes.POINTER(ctypes.c_int32)]),
+    ("mnl_plan_matrix_async", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp, ctypes.c_double, _vp, _vp,
+      ctypes.c_int32, _vp, ctypes.c_int, ctypes.c_int32, _vp, _vp, _vp, _vp]),
     ("mnl_plan_set_batch", ctypes.c_int,
      [_vp, _vp, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("mnl_plan_batch_count", ctypes.c_int, [_vp, _vp, _vp, _lp]),

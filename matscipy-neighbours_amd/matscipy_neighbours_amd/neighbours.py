@@ -26,6 +26,7 @@ from . import _lib
 __all__ = [
     "neighbour_list",
     "neighbour_list_batch",
+    "neighbour_matrix",
     "coordination",
     "first_neighbours",
     "get_jump_indicies",
@@ -472,6 +473,51 @@ def neighbour_list_batch(quantities, positions, cells, pbc, cutoff, natoms, *,
         _lib.check(lib.mnl_plan_system_offsets(plan.handle, sp, _lib.ptr(sys_off)))
     tensors = [out[q] for q in quantities] + [sys_off]
     return tuple(_export(tensors, want_dev, array_namespace))
+
+
+def neighbour_matrix(atoms=None, cutoff=None, max_neighbours=None, *, positions=None,
+                     cell=None, pbc=None, numbers=None, cell_origin=None, device=None,
+                     array_namespace=None):
+    """Dense fixed-capacity neighbour list ``(idx, dist, count)``
+    (neighbours.py:291-357 of the reference).
+
+    ``idx`` (n, max_neighbours) int32 holds each atom's neighbours in the
+    pair list's order (unused slots 0), ``dist`` (n, max_neighbours, 3) the
+    distance vectors ``D`` (unused slots 0), ``count`` (n,) the true
+    neighbour counts. Raises ``ValueError`` if any atom has more than
+    ``max_neighbours`` neighbours. Built in one traversal on the GPU (no
+    pair list)."""
+    if cutoff is None:
+        raise ValueError("Please provide a value for the cutoff radius.")
+    if max_neighbours is None:
+        raise ValueError("Please provide max_neighbours (the row capacity).")
+    K = int(max_neighbours)
+    pos_t, co, ce, inv, pb, nums, on_device, want_dev = _prepare(
+        atoms, positions, cell, pbc, numbers, cell_origin, device)
+    rc, nums = _resolve_cutoff(cutoff, nums)
+    b = _Build(pos_t, co, ce, inv, pb, rc, nums)
+    torch = _torch()
+    lib = _lib.load()
+    n = b.nat
+    idx = b.dev.index if b.dev.index is not None else torch.cuda.current_device()
+    plan = _lib.plan_for(idx)
+    kw = dict(device=b.dev)
+    m_idx = torch.empty((n, K), dtype=torch.int32, **kw)
+    m_dist = torch.empty((n, K, 3), dtype=torch.float64, **kw)
+    m_cnt = torch.empty(n, dtype=torch.int32, **kw)
+    overflow = _lib.ctypes.c_int32(0)
+    with plan.lock, torch.cuda.device(idx):
+        sp = _lib.ctypes.c_void_p(torch.cuda.current_stream(idx).cuda_stream)
+        _lib.check(lib.mnl_plan_matrix(
+            plan.handle, sp, _lib.ptr(b.co), _lib.ptr(b.ce), _lib.ptr(b.inv), _lib.ptr(b.pbc),
+            n, _lib.ptr(b.pos), b.rc, _lib.ptr(b.per_atom_t), _lib.ptr(b.per_type_sq), b.nc,
+            _lib.ptr(b.types_t), -1, K, _lib.ptr(m_idx), _lib.ptr(m_dist), _lib.ptr(m_cnt),
+            _lib.ctypes.byref(overflow)))
+    if overflow.value:
+        raise ValueError(
+            f"max_neighbours={K} is too small: some atom has more neighbours than that. "
+            "Retry with a larger max_neighbours.")
+    return tuple(_export([m_idx, m_dist, m_cnt], want_dev, array_namespace))
 
 
 def coordination(atoms=None, cutoff=None, *, positions=None, cell=None, pbc=None,

@@ -232,6 +232,32 @@ def first_neighbours(n, i):
     return seed
 
 
+def neighbour_matrix(cell_origin, cell, inv_cell, pbc, positions, cutoff, K, types=None):
+    """Dense neighbour matrix, restating src/libneighbours/neighbour_list.cc:206-254:
+    the pair list (i, j, D) reshaped into rows of K slots in list order, unused
+    slots 0, count = row length, overflow if some row is longer than K.
+    Returns (idx (nat, K) int32, dist (nat, K, 3) float64, count (nat,) int32,
+    overflow bool)."""
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    nat = pos.shape[0]
+    idx = np.zeros((nat, K), np.int32)
+    dist = np.zeros((nat, K, 3), np.float64)
+    cnt = np.zeros(nat, np.int32)
+    if nat == 0:
+        return idx, dist, cnt, False
+    nl = neighbour_list("ijD", cell_origin, cell, inv_cell, pbc, pos, cutoff, types=types)
+    i, j, D = nl["i"], nl["j"], nl["D"]
+    off = np.zeros(nat + 1, np.int64)
+    np.add.at(off, i + 1, 1)
+    off = np.cumsum(off)
+    cnt[:] = np.diff(off)
+    for a in range(nat):
+        m = min(int(cnt[a]), K)
+        idx[a, :m] = j[off[a]:off[a] + m]
+        dist[a, :m] = D[off[a]:off[a] + m]
+    return idx, dist, cnt, bool(np.any(cnt > K))
+
+
 def ref_neighbour_list(quantities, cell_origin, cell, inv_cell, pbc, positions, cutoff,
                        types=None):
     """The compiled reference core (oracle/_ref), same argument handling."""

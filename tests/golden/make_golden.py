@@ -190,6 +190,28 @@ def main():
         {"first_i": first_i, "absdist": absd, "cutoff": 2.6, "ij": ij2.tolist(),
          "ik": ik2.tolist()}]
 
+    # dense neighbour matrix known answers (neighbour_list.cc:206-254)
+    mats = []
+    for mname, K in (("cubic_random_TTT", 16), ("triclinic_outside_TTF", 12),
+                     ("per_type_out_of_range", 8), ("tiny_cell_rc2.5_TTT", 64),
+                     ("uniform_1000_rc2.5", 96), ("uniform_1000_rc2.5", 40)):
+        pos = arrays[f"{mname}/positions"]
+        cell = arrays[f"{mname}/cell"]
+        inv = arrays[f"{mname}/inv_cell"]
+        pbc = arrays[f"{mname}/pbc"]
+        cut = arrays[f"{mname}/cutoff"]
+        cutoff = float(cut) if cut.ndim == 0 else cut
+        types = arrays.get(f"{mname}/types")
+        idx, dist, cnt, ovf = oracle.ref_neighbour_matrix(np.zeros(3), cell, inv, pbc, pos,
+                                                          cutoff, K, types=types)
+        key = f"matrix/{mname}/K{K}"
+        arrays[f"{key}/idx"] = idx
+        arrays[f"{key}/dist"] = dist
+        arrays[f"{key}/count"] = cnt
+        mats.append({"case": mname, "K": K, "overflow": bool(ovf)})
+        print(f"matrix {mname:28s} K={K:3d} overflow={ovf}")
+    post["neighbour_matrix"] = mats
+
     np.savez_compressed(os.path.join(HERE, "neighbour_list_golden.npz"), **arrays)
     with open(os.path.join(HERE, "neighbour_list_golden.json"), "w") as f:
         json.dump({"source": "reference CPU core (src/libneighbours, CellOrder::Linear) "

@@ -241,3 +241,19 @@ def test_post_processing_known_answers():
     ij, ik = triplet_list(np.array(t1["first_i"], np.int32), np.array(t1["absdist"]),
                           t1["cutoff"])
     assert ij.tolist() == t1["ij"] and ik.tolist() == t1["ik"]
+
+
+def test_oracle_matrix_matches_golden():
+    """The dense-matrix restatement against the compiled reference's
+    neighbour_matrix (neighbour_list.cc:206-254), bit for bit, overflow flag
+    included."""
+    for c in INDEX["post"]["neighbour_matrix"]:
+        name, K = c["case"], c["K"]
+        pos, cell, inv, pbc, cutoff, types = case_inputs(name)
+        idx, dist, cnt, ovf = oracle.neighbour_matrix(np.zeros(3), cell, inv, pbc, pos, cutoff,
+                                                      K, types=types)
+        key = f"matrix/{name}/K{K}"
+        assert np.array_equal(idx, ARR[f"{key}/idx"]), key
+        assert np.array_equal(bits(dist), bits(ARR[f"{key}/dist"])), key
+        assert np.array_equal(cnt, ARR[f"{key}/count"]), key
+        assert ovf == c["overflow"], key
