@@ -329,6 +329,37 @@ int mnl_plan_pair_backward(mnl_plan *plan, void *stream, int64_t npairs, int32_t
                            const void *grad_absdist, int float32,
                            double *grad_positions, double *grad_cell);
 
+/* ---------------------------------------------------------------- lists
+ * Post-processing of i-sorted pair lists, all on the device.
+ */
+
+/* Row starts of an i-sorted list: seed [n+1] int32 [device] from the first
+   indices i [nn] int32 [device] (asynchronous on `stream`). Rows before the
+   first atom with pairs get -1, rows after the last get nn, rows without
+   pairs point at the next row's start; nn == 0 gives all 0. Replaces
+   matscipy::first_neighbours (src/libneighbours/first_neighbours.cc:15-40). */
+int mnl_first_neighbours(void *stream, int32_t n, int64_t nn, const int32_t *i,
+                         int32_t *seed);
+
+/* Number of distinct values of a sorted int32 array [device] (waits): the n
+   that get_jump_indicies (first_neighbours.cc:42-55) passes to
+   first_neighbours, 1 for an empty array. */
+int mnl_plan_count_distinct(mnl_plan *plan, void *stream, int64_t nn, const int32_t *sorted,
+                            int32_t *ndistinct);
+
+/* Triplets (pairs of pairs sharing the first atom) of a first-neighbour
+   array, as matscipy::triplet_list (src/libneighbours/triplet_list.cc:15-34):
+   for each row r < n_first - 1 every ordered (ij, ik), ij != ik, of pair
+   indices in [first_i[r], first_i[r+1]), ij outer and ik inner, both
+   ascending; with absdist [nn] (may be NULL) only pairs with
+   !(absdist >= cutoff). Phase 1 waits and returns the triplet count; phase 2
+   writes ij, ik [ntriplets] int32 [device]. first_i [n_first] int32
+   [device] must stay valid until phase 2. */
+int mnl_plan_triplets_count(mnl_plan *plan, void *stream, int32_t n_first,
+                            const int32_t *first_i, int64_t nn, const double *absdist,
+                            double cutoff, int64_t *ntriplets);
+int mnl_plan_triplets_fill(mnl_plan *plan, void *stream, int32_t *ij, int32_t *ik);
+
 /* Diagnostic, no counterpart in the reference: y = sqrt(x) for n doubles
    (host buffers) computed by a device kernel, so that tests can pin that the
    device square root rounds exactly like the host's (d == sqrt(r2)). */

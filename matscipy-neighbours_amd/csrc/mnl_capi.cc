@@ -122,7 +122,7 @@ struct mnl_plan {
     /* workspaces */
     DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, counts,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
-        htypes, cell_partial;
+        htypes, cell_partial, tri_m, tri_cnt, tri_off, tri_vidx, tri_flag;
     int64_t *host_total = nullptr; /* pinned */
     /* state of the last count */
     bool counted = false;
@@ -137,6 +137,10 @@ struct mnl_plan {
     hipStream_t stream = nullptr;
     /* host copy of the per-type table on the device */
     std::vector<double> pt_host;
+    /* triplets: rows and pairs of the last mnl_plan_triplets_count */
+    int32_t tri_rows = 0;
+    int64_t tri_total = -1;
+    const int32_t *tri_first = nullptr;
     /* batched geometry uploaded by mnl_plan_set_batch and still in place */
     bool batch_ready = false;
 };
@@ -596,7 +600,8 @@ void mnl_plan_destroy(mnl_plan *p) {
                       &p->stype,     &p->cell_count, &p->cell_first, &p->counts, &p->offsets,
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
                       &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
-                      &p->cell_partial};
+                      &p->cell_partial, &p->tri_m,   &p->tri_cnt,  &p->tri_off,
+                      &p->tri_vidx,  &p->tri_flag};
     for (DevBuf *b : bufs) b->release();
     if (p->host_total) (void)hipHostFree(p->host_total);
     delete p;
@@ -1069,6 +1074,90 @@ int mnl_plan_pair_backward(mnl_plan *p, void *stream, int64_t npairs, int32_t na
     MNL_HIP(launch_pair_backward(npairs, nat, first, secnd, shift, distvec, absdist,
                                  grad_distvec, grad_absdist, float32 != 0, grad_positions,
                                  grad_cell, p->cell_partial.as<double>(), st));
+    return MNL_SUCCESS;
+}
+
+int mnl_first_neighbours(void *stream, int32_t n, int64_t nn, const int32_t *i, int32_t *seed) {
+    if (n < 0 || nn < 0) return fail("negative size");
+    if (nn > INT32_MAX) return fail("first_neighbours: more than 2^31 - 1 pairs");
+    if (!seed || (nn > 0 && !i)) return fail("first_neighbours: NULL pointer");
+    if (check_device_available() != MNL_SUCCESS) return MNL_ERROR;
+    MNL_HIP(launch_first_neighbours(i, nn, n, seed, static_cast<hipStream_t>(stream)));
+    return MNL_SUCCESS;
+}
+
+int mnl_plan_count_distinct(mnl_plan *p, void *stream, int64_t nn, const int32_t *sorted,
+                            int32_t *ndistinct) {
+    if (!p) return fail("plan is NULL");
+    if (!ndistinct) return fail("ndistinct is NULL");
+    if (nn > 0 && !sorted) return fail("sorted is NULL");
+    DeviceScope ds(p->device);
+    if (!ds.ok) return fail("cannot switch to the plan's device");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (p->tri_flag.cap < sizeof(int64_t)) {
+        MNL_HIP(hipStreamSynchronize(st));
+        MNL_HIP(p->tri_flag.ensure(sizeof(int64_t)));
+    }
+    MNL_HIP(launch_count_jumps(sorted, nn, p->tri_flag.as<int32_t>(), st));
+    MNL_HIP(hipMemcpyAsync(p->host_total, p->tri_flag.ptr, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, st));
+    MNL_HIP(hipStreamSynchronize(st));
+    int32_t jumps = 0;
+    std::memcpy(&jumps, p->host_total, sizeof jumps);
+    *ndistinct = jumps + 1; /* get_jump_indicies: jumps + 1, also for nn == 0 */
+    return MNL_SUCCESS;
+}
+
+int mnl_plan_triplets_count(mnl_plan *p, void *stream, int32_t n_first, const int32_t *first_i,
+                            int64_t nn, const double *absdist, double cutoff,
+                            int64_t *ntriplets) {
+    if (!p) return fail("plan is NULL");
+    if (!ntriplets) return fail("ntriplets is NULL");
+    if (n_first < 0 || nn < 0) return fail("negative size");
+    if (n_first > 0 && !first_i) return fail("first_i is NULL");
+    DeviceScope ds(p->device);
+    if (!ds.ok) return fail("cannot switch to the plan's device");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    p->tri_total = -1;
+    const int32_t rows = n_first > 1 ? n_first - 1 : 0;
+    const size_t r1 = static_cast<size_t>(rows) + 1;
+    struct {
+        DevBuf *b;
+        size_t bytes;
+    } needs[] = {{&p->tri_m, r1 * sizeof(int32_t)},
+                 {&p->tri_cnt, r1 * sizeof(int64_t)},
+                 {&p->tri_off, (r1 + 1) * sizeof(int64_t)},
+                 {&p->tri_vidx, std::max<size_t>(1, static_cast<size_t>(nn)) * sizeof(int32_t)},
+                 {&p->tri_flag, sizeof(int64_t)},
+                 {&p->partial, static_cast<size_t>(scan_partial_size(rows)) * sizeof(int64_t)}};
+    bool grow = false;
+    for (auto &nd : needs) grow = grow || nd.bytes > nd.b->cap;
+    if (grow) {
+        MNL_HIP(hipStreamSynchronize(st));
+        for (auto &nd : needs) MNL_HIP(nd.b->ensure(nd.bytes));
+    }
+    MNL_HIP(launch_triplet_rows(first_i, rows, nn, absdist, cutoff, p->tri_m.as<int32_t>(),
+                                p->tri_cnt.as<int64_t>(), p->tri_vidx.as<int32_t>(), st));
+    MNL_HIP(launch_scan_i64_to_i64(p->tri_cnt.as<int64_t>(), rows, p->tri_off.as<int64_t>(),
+                                   p->partial.as<int64_t>(), p->tri_flag.as<int64_t>(), st));
+    MNL_HIP(hipMemcpyAsync(p->host_total, p->tri_flag.ptr, sizeof(int64_t),
+                           hipMemcpyDeviceToHost, st));
+    MNL_HIP(hipStreamSynchronize(st));
+    p->tri_total = *p->host_total;
+    p->tri_rows = rows;
+    p->tri_first = first_i;
+    *ntriplets = p->tri_total;
+    return MNL_SUCCESS;
+}
+
+int mnl_plan_triplets_fill(mnl_plan *p, void *stream, int32_t *ij, int32_t *ik) {
+    if (!p) return fail("plan is NULL");
+    if (p->tri_total < 0) return fail("mnl_plan_triplets_fill without mnl_plan_triplets_count");
+    if (p->tri_total > 0 && (!ij || !ik)) return fail("triplet output is NULL");
+    DeviceScope ds(p->device);
+    MNL_HIP(launch_triplet_fill(p->tri_first, p->tri_rows, p->tri_m.as<int32_t>(),
+                                p->tri_off.as<int64_t>(), p->tri_vidx.as<int32_t>(), ij, ik,
+                                static_cast<hipStream_t>(stream)));
     return MNL_SUCCESS;
 }
 

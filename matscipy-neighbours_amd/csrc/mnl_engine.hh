@@ -116,6 +116,23 @@ hipError_t launch_scan_i32_to_i32(const int32_t *in, int64_t n, int32_t *out,
 hipError_t launch_scan_i32_to_i64(const int32_t *in, int64_t n, int64_t *out,
                                   int64_t *partial, int64_t *total,
                                   hipStream_t stream);
+hipError_t launch_scan_i64_to_i64(const int64_t *in, int64_t n, int64_t *out,
+                                  int64_t *partial, int64_t *total,
+                                  hipStream_t stream);
+
+/* Post-processing of i-sorted pair lists (mnl_post.hip). */
+hipError_t launch_first_neighbours(const int32_t *i, int64_t nn, int32_t n, int32_t *seed,
+                                   hipStream_t stream);
+hipError_t launch_count_jumps(const int32_t *sorted, int64_t nn, int32_t *count,
+                              hipStream_t stream);
+/* rows r < nrows (= n_first - 1): m[r] kept pairs, tcnt[r] = m (m - 1),
+   vidx[first_i[r] + k] = k-th kept pair of the row (row ends clamped to nn) */
+hipError_t launch_triplet_rows(const int32_t *first_i, int32_t nrows, int64_t nn,
+                               const double *absdist, double cutoff, int32_t *m, int64_t *tcnt,
+                               int32_t *vidx, hipStream_t stream);
+hipError_t launch_triplet_fill(const int32_t *first_i, int32_t nrows, const int32_t *m,
+                               const int64_t *toff, const int32_t *vidx, int32_t *ij,
+                               int32_t *ik, hipStream_t stream);
 /* Pair kernels (one wave per home cell). mode 0 = count, 1 = fill f64
    outputs, 2 = fill f32 outputs, 3 = dense matrix (rows of max_nb slots:
    out_j = neighbour index, out_D = D, counts = row counts; no count pass). */

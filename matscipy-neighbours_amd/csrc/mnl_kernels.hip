@@ -1123,8 +1123,8 @@ hipError_t launch_rank_gather(const double *r, const int4 *raw4, const int32_t *
 
 int64_t scan_partial_size(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
-template <typename TO>
-static hipError_t scan_impl(const int32_t *in, int64_t n, TO *out, int64_t *partial,
+template <typename TI, typename TO>
+static hipError_t scan_impl(const TI *in, int64_t n, TO *out, int64_t *partial,
                             int64_t *total, bool write_total, hipStream_t stream) {
     if (n <= 0) {
         if (write_total) {
@@ -1138,21 +1138,26 @@ static hipError_t scan_impl(const int32_t *in, int64_t n, TO *out, int64_t *part
         return hipGetLastError();
     }
     const int64_t nb = (n + kScanTile - 1) / kScanTile;
-    k_scan_reduce<int32_t><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(in, n, partial);
+    k_scan_reduce<TI><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(in, n, partial);
     k_scan_spine<<<1, 1024, 0, stream>>>(partial, nb, total);
-    k_scan_apply<int32_t, TO><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(
+    k_scan_apply<TI, TO><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(
         in, n, partial, out, write_total ? 1 : 0);
     return hipGetLastError();
 }
 
 hipError_t launch_scan_i32_to_i32(const int32_t *in, int64_t n, int32_t *out,
                                   int64_t *partial, int64_t *total, hipStream_t stream) {
-    return scan_impl<int32_t>(in, n, out, partial, total, false, stream);
+    return scan_impl<int32_t, int32_t>(in, n, out, partial, total, false, stream);
 }
 
 hipError_t launch_scan_i32_to_i64(const int32_t *in, int64_t n, int64_t *out,
                                   int64_t *partial, int64_t *total, hipStream_t stream) {
-    return scan_impl<int64_t>(in, n, out, partial, total, true, stream);
+    return scan_impl<int32_t, int64_t>(in, n, out, partial, total, true, stream);
+}
+
+hipError_t launch_scan_i64_to_i64(const int64_t *in, int64_t n, int64_t *out,
+                                  int64_t *partial, int64_t *total, hipStream_t stream) {
+    return scan_impl<int64_t, int64_t>(in, n, out, partial, total, true, stream);
 }
 
 template <int CUT, bool HASHED>

@@ -537,57 +537,86 @@ def coordination(atoms=None, cutoff=None, *, positions=None, cell=None, pbc=None
     return _export([counts], want_dev, array_namespace)[0]
 
 
+def _int32_on_device(x):
+    """(tensor on the GPU, was_device) for an int32 index array."""
+    torch = _torch()
+    if _is_on_device(x):
+        t = x if isinstance(x, torch.Tensor) else torch.from_dlpack(x)
+        return t.to(torch.int32).contiguous(), True
+    a = np.ascontiguousarray(x, dtype=np.int32).reshape(-1)
+    return torch.from_numpy(a).to(f"cuda:{torch.cuda.current_device()}"), False
+
+
 def first_neighbours(n, i):
     """Row-start index of each atom in an i-sorted pair list
-    (first_neighbours.cc:15-40); -1 for atoms before the first populated row."""
-    i = np.ascontiguousarray(i, dtype=np.int32)
+    (first_neighbours.cc:15-40): seed (n + 1,) int32; -1 for atoms before the
+    first populated row, nn after the last. Computed on the GPU; host input
+    gives a host result."""
+    _lib.require_gpu()
+    torch = _torch()
+    lib = _lib.load()
     n = int(n)
-    seed = np.full(n + 1, -1, dtype=np.int32)
-    nn = i.shape[0]
-    if nn == 0:
-        seed[:] = 0
-        return seed
-    seed[i[0]] = 0
-    jumps = np.nonzero(i[1:] != i[:-1])[0] + 1
-    for k in jumps:
-        seed[i[k - 1] + 1:i[k] + 1] = k
-    seed[i[-1] + 1:] = nn
-    return seed
+    it, dev_in = _int32_on_device(i)
+    seed = torch.empty(n + 1, dtype=torch.int32, device=it.device)
+    with torch.cuda.device(it.device):
+        sp = _lib.ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.check(lib.mnl_first_neighbours(sp, n, int(it.numel()), _lib.ptr(it),
+                                            _lib.ptr(seed)))
+    return seed if dev_in else seed.cpu().numpy()
 
 
 def get_jump_indicies(sorted_array):
     """Positions where a sorted array changes value, plus both ends
-    (first_neighbours.cc:42-55)."""
-    s = np.ascontiguousarray(sorted_array, dtype=np.int32)
-    nn = s.shape[0]
-    n = int(np.count_nonzero(s[1:] != s[:-1])) + 1 if nn > 0 else 1
-    return first_neighbours(n, s)
+    (first_neighbours.cc:42-55). Computed on the GPU."""
+    _lib.require_gpu()
+    torch = _torch()
+    lib = _lib.load()
+    it, dev_in = _int32_on_device(sorted_array)
+    plan = _lib.plan_for(it.device.index)
+    nd = _lib.ctypes.c_int32(0)
+    with plan.lock, torch.cuda.device(it.device):
+        sp = _lib.ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.check(lib.mnl_plan_count_distinct(plan.handle, sp, int(it.numel()), _lib.ptr(it),
+                                               _lib.ctypes.byref(nd)))
+    out = first_neighbours(int(nd.value), it)
+    return out if dev_in else out.cpu().numpy()
 
 
 def triplet_list(first_neighbours, abs_dr_p=None, cutoff=None):
     """Triplets (ij, ik) of pairs sharing the first atom (triplet_list.cc:15-34,
-    neighbours.py:360-370)."""
-    fi = np.ascontiguousarray(first_neighbours, dtype=np.int32)
+    neighbours.py:360-370 of the reference): computed on the GPU; host
+    input gives host arrays."""
     if (abs_dr_p is None) != (cutoff is None):
         raise TypeError("Cutoff and distances must be specified together.")
     if abs_dr_p is not None and not isinstance(cutoff, float):
         raise NotImplementedError("Cutoff must be a single float.")
-    d = None if abs_dr_p is None else np.ascontiguousarray(abs_dr_p, dtype=float)
-    ij_t, ik_t = [], []
-    for r in range(fi.shape[0] - 1):
-        rows = np.arange(fi[r], fi[r + 1], dtype=np.int32)
-        if rows.size == 0:
-            continue
-        a = np.repeat(rows, rows.size)
-        b = np.tile(rows, rows.size)
-        keep = a != b
-        if d is not None:
-            keep &= (d[a] < cutoff) & (d[b] < cutoff)
-        ij_t.append(a[keep])
-        ik_t.append(b[keep])
-    if not ij_t:
-        return np.zeros(0, np.int32), np.zeros(0, np.int32)
-    return np.concatenate(ij_t).astype(np.int32), np.concatenate(ik_t).astype(np.int32)
+    _lib.require_gpu()
+    torch = _torch()
+    lib = _lib.load()
+    ft, dev_in = _int32_on_device(first_neighbours)
+    d = None
+    if abs_dr_p is not None:
+        if _is_on_device(abs_dr_p):
+            d = abs_dr_p if isinstance(abs_dr_p, torch.Tensor) else torch.from_dlpack(abs_dr_p)
+            d = d.to(device=ft.device, dtype=torch.float64).contiguous()
+        else:
+            d = torch.from_numpy(np.ascontiguousarray(abs_dr_p, dtype=np.float64)).to(ft.device)
+    nf = int(ft.numel())
+    nn = int(d.numel()) if d is not None else (int(ft[-1].item()) if nf else 0)
+    nn = max(nn, 0)
+    plan = _lib.plan_for(ft.device.index)
+    nt = _lib.ctypes.c_int64(0)
+    with plan.lock, torch.cuda.device(ft.device):
+        sp = _lib.ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.check(lib.mnl_plan_triplets_count(plan.handle, sp, nf, _lib.ptr(ft), nn,
+                                               _lib.ptr(d), float(cutoff or 0.0),
+                                               _lib.ctypes.byref(nt)))
+        ij = torch.empty(int(nt.value), dtype=torch.int32, device=ft.device)
+        ik = torch.empty(int(nt.value), dtype=torch.int32, device=ft.device)
+        _lib.check(lib.mnl_plan_triplets_fill(plan.handle, sp, _lib.ptr(ij), _lib.ptr(ik)))
+    if dev_in:
+        return ij, ik
+    return ij.cpu().numpy(), ik.cpu().numpy()
 
 
 _lock = threading.Lock()

@@ -227,9 +227,42 @@ def grid_geometry(cell, cutoff):
 def first_neighbours(n, i):
     lib = _load_oracle()
     i = np.ascontiguousarray(i, dtype=np.int32)
+    # the C routine indexes seed[i[k]] (as the reference does): values outside
+    # [0, n] would write out of bounds
+    if i.size and (i.min() < 0 or i.max() > n):
+        raise ValueError("first_neighbours: index outside [0, n]")
     seed = np.zeros(n + 1, np.int32)
     lib.oracle_first_neighbours(int(n), int(i.shape[0]), _ptr(i, "i"), _ptr(seed, "i"))
     return seed
+
+
+def get_jump_indicies(sorted_arr):
+    """src/libneighbours/first_neighbours.cc:42-55: the number of distinct
+    values (jumps + 1), then first_neighbours of that many rows."""
+    s = np.ascontiguousarray(sorted_arr, dtype=np.int32)
+    n = 1
+    for k in range(len(s) - 1):
+        if s[k] != s[k + 1]:
+            n += 1
+    return first_neighbours(n, s)
+
+
+def triplet_list(first_i, absdist=None, cutoff=0.0):
+    """src/libneighbours/triplet_list.cc:15-34 (pure-Python loops: small
+    inputs only): per row, every (ij, ik) with ij != ik, ij outer, ik inner,
+    skipping pairs with absdist >= cutoff."""
+    f = np.ascontiguousarray(first_i, dtype=np.int64)
+    ij_t, ik_t = [], []
+    for r in range(len(f) - 1):
+        for ij in range(f[r], f[r + 1]):
+            for ik in range(f[r], f[r + 1]):
+                if ij == ik:
+                    continue
+                if absdist is not None and (absdist[ij] >= cutoff or absdist[ik] >= cutoff):
+                    continue
+                ij_t.append(ij)
+                ik_t.append(ik)
+    return np.array(ij_t, np.int32), np.array(ik_t, np.int32)
 
 
 def neighbour_matrix(cell_origin, cell, inv_cell, pbc, positions, cutoff, K, types=None):

@@ -1,0 +1,72 @@
+"""first_neighbours, get_jump_indicies and triplet_list on the GPU against
+the compiled reference's known answers (tests/test_extension.py:277-337)
+and the oracle's restatements on real pair lists, host and device inputs."""
+
+import numpy as np
+import pytest
+
+import oracle
+from test_oracle import INDEX
+
+pytestmark = pytest.mark.gpu
+
+
+def test_known_answers(gpu):
+    from matscipy_neighbours_amd import first_neighbours, get_jump_indicies, triplet_list
+    post = INDEX["post"]
+    for c in post["first_neighbours"]:
+        assert first_neighbours(c["n"], np.array(c["i"], np.int32)).tolist() == c["seed"]
+    for c in post["get_jump_indicies"]:
+        assert get_jump_indicies(np.array(c["sorted"], np.int32)).tolist() == c["out"]
+    t0, t1 = post["triplet_list"]
+    ij, ik = triplet_list(np.array(t0["first_i"], np.int32))
+    assert ij.tolist() == t0["ij"] and ik.tolist() == t0["ik"]
+    ij, ik = triplet_list(np.array(t1["first_i"], np.int32), np.array(t1["absdist"]),
+                          t1["cutoff"])
+    assert ij.tolist() == t1["ij"] and ik.tolist() == t1["ik"]
+
+
+def test_on_pair_lists(gpu):
+    """Real pair lists (atoms with no neighbours included), host and device."""
+    import torch
+
+    from matscipy_neighbours_amd import (first_neighbours, get_jump_indicies,
+                                         neighbour_list, triplet_list)
+    rng = np.random.default_rng(12)
+    pos = rng.uniform(0, 14, (400, 3))
+    pos[:40] += 100.0  # isolated atoms far outside (no neighbours)
+    for nonperiodic in (False, True):
+        i, d = neighbour_list("id", positions=pos, cell=14 * np.eye(3), pbc=not nonperiodic,
+                              cutoff=2.0)
+        seed = first_neighbours(len(pos), i)
+        want = oracle.first_neighbours(len(pos), i)
+        assert np.array_equal(seed, want)
+        # get_jump_indicies takes values 0 .. n-1 (it indexes seed[value]):
+        # relabel the atoms that have pairs densely
+        dense = np.unique(i, return_inverse=True)[1].astype(np.int32)
+        assert np.array_equal(get_jump_indicies(dense), oracle.get_jump_indicies(dense))
+        for args in ((seed,), (seed, d, 1.5)):
+            got = triplet_list(*args)
+            ref = oracle.triplet_list(*args)
+            assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+        # device in, device out
+        it = torch.from_numpy(i).to("cuda:0")
+        sd = first_neighbours(len(pos), it)
+        assert isinstance(sd, torch.Tensor) and np.array_equal(sd.cpu().numpy(), want)
+        ij, ik = triplet_list(sd, torch.from_numpy(d).to("cuda:0"), 1.5)
+        ref = oracle.triplet_list(seed, d, 1.5)
+        assert np.array_equal(ij.cpu().numpy(), ref[0]) and np.array_equal(ik.cpu().numpy(),
+                                                                           ref[1])
+
+
+def test_empty_and_edge_cases(gpu):
+    from matscipy_neighbours_amd import first_neighbours, get_jump_indicies, triplet_list
+    assert first_neighbours(4, np.zeros(0, np.int32)).tolist() == [0] * 5
+    assert get_jump_indicies(np.zeros(0, np.int32)).tolist() == \
+        oracle.get_jump_indicies(np.zeros(0, np.int32)).tolist()
+    ij, ik = triplet_list(np.array([0, 1, 1, 3], np.int32))
+    assert ij.tolist() == [1, 2] and ik.tolist() == [2, 1]
+    d = np.array([0.5, 1.0, np.nan])  # NaN is kept (not >= cutoff)
+    ij, ik = triplet_list(np.array([0, 3], np.int32), d, 0.9)
+    ref = oracle.triplet_list(np.array([0, 3], np.int32), d, 0.9)
+    assert ij.tolist() == ref[0].tolist() and ik.tolist() == ref[1].tolist()

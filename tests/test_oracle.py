@@ -226,21 +226,35 @@ def test_planted_mistake_is_detected(name, perturb):
 
 
 def test_post_processing_known_answers():
-    """first_neighbours / get_jump_indicies / triplet_list of the package
-    against the compiled reference's answers (tests/test_extension.py:277-337)."""
-    from matscipy_neighbours_amd import first_neighbours, get_jump_indicies, triplet_list
+    """The oracle's first_neighbours / get_jump_indicies / triplet_list against
+    the compiled reference's answers (tests/test_extension.py:277-337)."""
     post = INDEX["post"]
     for c in post["first_neighbours"]:
-        assert first_neighbours(c["n"], np.array(c["i"], np.int32)).tolist() == c["seed"]
         assert oracle.first_neighbours(c["n"], np.array(c["i"], np.int32)).tolist() == c["seed"]
     for c in post["get_jump_indicies"]:
-        assert get_jump_indicies(np.array(c["sorted"], np.int32)).tolist() == c["out"]
+        assert oracle.get_jump_indicies(np.array(c["sorted"], np.int32)).tolist() == c["out"]
     t0, t1 = post["triplet_list"]
-    ij, ik = triplet_list(np.array(t0["first_i"], np.int32))
+    ij, ik = oracle.triplet_list(np.array(t0["first_i"], np.int32))
     assert ij.tolist() == t0["ij"] and ik.tolist() == t0["ik"]
-    ij, ik = triplet_list(np.array(t1["first_i"], np.int32), np.array(t1["absdist"]),
-                          t1["cutoff"])
+    ij, ik = oracle.triplet_list(np.array(t1["first_i"], np.int32), np.array(t1["absdist"]),
+                                 t1["cutoff"])
     assert ij.tolist() == t1["ij"] and ik.tolist() == t1["ik"]
+
+
+def test_oracle_triplets_vs_compiled_reference():
+    """Random first-neighbour arrays with distances: the restatement equals the
+    compiled reference (where it was built)."""
+    if not oracle.ref_available():
+        pytest.skip("compiled reference not built")
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        counts = rng.integers(0, 7, int(rng.integers(1, 12)))
+        first = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+        d = rng.uniform(0, 3, int(first[-1]))
+        for args in ((first,), (first, d, 1.7)):
+            want = oracle.ref_triplet_list(*args)
+            got = oracle.triplet_list(*args)
+            assert np.array_equal(want[0], got[0]) and np.array_equal(want[1], got[1])
 
 
 def test_oracle_matrix_matches_golden():
