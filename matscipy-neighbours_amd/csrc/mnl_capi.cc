@@ -120,7 +120,7 @@ bool valid_quantities(int q) { return (q & ~31) == 0; }
 struct mnl_plan {
     int device = 0;
     /* workspaces */
-    DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, counts,
+    DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, scount, counts,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
         htypes, cell_partial, tri_m, tri_cnt, tri_off, tri_vidx, tri_flag;
     int64_t *host_total = nullptr; /* pinned */
@@ -137,6 +137,7 @@ struct mnl_plan {
     hipStream_t stream = nullptr;
     /* host copy of the per-type table on the device */
     std::vector<double> pt_host;
+    bool pt_symmetric = true; /* pt_host[a][b] == pt_host[b][a] for all a, b */
     /* triplets: rows and pairs of the last mnl_plan_triplets_count */
     int32_t tri_rows = 0;
     int64_t tri_total = -1;
@@ -164,6 +165,10 @@ PairArgs pair_args(const mnl_plan *p) {
     a.pt_sq = p->cut == CUT_PER_TYPE ? p->pt.as<double>() : nullptr;
     a.ncut = p->ncut;
     a.atom_base = p->atom_base.as<int32_t>();
+    a.nat = p->nat;
+    /* the count visits half the offsets when the test is symmetric in i, j */
+    a.half = (p->cut != CUT_PER_TYPE || p->ncut == 0 || p->pt_symmetric) ? 1 : 0;
+    a.scount = p->scount.as<int32_t>();
     a.counts = p->counts.as<int32_t>();
     a.offsets = p->offsets.as<int64_t>();
     a.capacity = p->npairs;
@@ -194,6 +199,7 @@ int ensure_workspaces(mnl_plan *p, hipStream_t stream, int32_t nat, int64_t cell
         {&p->pt, cut == CUT_PER_TYPE
                      ? sizeof(double) * std::max<int64_t>(1, int64_t(ncutoffs) * ncutoffs)
                      : 0},
+        {&p->scount, n * sizeof(int32_t)},
         {&p->counts, n * sizeof(int32_t)},
         {&p->offsets, (n + 1) * sizeof(int64_t)},
         {&p->cell_count, static_cast<size_t>(std::max<int64_t>(cells, 1)) * sizeof(int32_t)},
@@ -251,7 +257,7 @@ int enqueue_binning(mnl_plan *p, hipStream_t stream, const double *positions,
                                p->geo.as<SysGeom>(), p->atom_base.as<int32_t>(), p->nsys,
                                per_atom, types, p->spos.as<double4>(), p->sinfo.as<int4>(),
                                p->cut == CUT_PER_TYPE ? p->stype.as<int32_t>() : nullptr,
-                               stream));
+                               p->scount.as<int32_t>(), stream));
     return MNL_SUCCESS;
 }
 
@@ -345,6 +351,12 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
                 return fail("a new per-type cutoff table cannot be uploaded under graph "
                             "capture: run one uncaptured build with it first");
             p->pt_host.assign(per_type_sq, per_type_sq + nt);
+            p->pt_symmetric = true;
+            for (int32_t u = 0; u < ncutoffs; u++)
+                for (int32_t v = 0; v < u; v++)
+                    if (!(p->pt_host[size_t(u) * ncutoffs + v] ==
+                          p->pt_host[size_t(v) * ncutoffs + u]))
+                        p->pt_symmetric = false;
             MNL_HIP(hipMemcpyAsync(p->pt.ptr, p->pt_host.data(), nt * sizeof(double),
                                    hipMemcpyHostToDevice, stream));
             MNL_HIP(hipStreamSynchronize(stream));
@@ -597,7 +609,7 @@ void mnl_plan_destroy(mnl_plan *p) {
     if (!p) return;
     DeviceScope ds(p->device);
     DevBuf *bufs[] = {&p->raw4,      &p->ticket,    &p->tmp,    &p->spos,     &p->sinfo,
-                      &p->stype,     &p->cell_count, &p->cell_first, &p->counts, &p->offsets,
+                      &p->stype,     &p->cell_count, &p->cell_first, &p->scount, &p->counts, &p->offsets,
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
                       &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
                       &p->cell_partial, &p->tri_m,   &p->tri_cnt,  &p->tri_off,

@@ -10,6 +10,7 @@
  *                      the image count (non-periodic), + atom index (system-local)
  *   stype  int32  [N]  cell-sorted atom type (per-type cutoffs only)
  *   cell_count / cell_first int32 [C] dense CSR (or hash slots when hashed)
+ *   scount int32  [N]  neighbours per atom, cell-sorted order (half count)
  *   counts int32  [N]  neighbours per atom, original order
  *   offsets int64 [N+1] exclusive scan of counts (CSR row pointers)
  * A "system" is one periodic cell with its atoms; the plain path is a batch
@@ -74,6 +75,9 @@ struct PairArgs {
     int32_t ncut;
     int32_t xcd_blocks;        /* grid size, for the XCD-aware block remap */
     const int32_t *atom_base;  /* [nsys+1] */
+    int32_t nat;               /* atoms of all systems */
+    int32_t half;              /* count: visit half the offsets (symmetric test) */
+    int32_t *scount;           /* half count: [N] by cell-sorted index, zeroed */
     int32_t *counts;           /* count: [N] by global atom index */
     const int64_t *offsets;    /* fill: [N+1] */
     int64_t capacity;          /* fill: pairs beyond this are not written */
@@ -105,7 +109,7 @@ hipError_t launch_rank_gather(const double *r, const int4 *raw4, const int32_t *
                               const int32_t *atom_base, int32_t nsys,
                               const double *per_atom, const int32_t *types,
                               double4 *spos, int4 *sinfo, int32_t *stype,
-                              hipStream_t stream);
+                              int32_t *scount, hipStream_t stream);
 /* Exclusive scan of n int32 values; out gets n (+1 with the total) entries.
    partial needs scan_partial_size(n) int64 entries. total (device, may be
    NULL) receives the sum. */

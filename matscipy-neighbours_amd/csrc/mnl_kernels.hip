@@ -229,7 +229,8 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
     const int32_t *__restrict__ cell_count, const SysGeom *__restrict__ geo,
     const int32_t *__restrict__ atom_base, int32_t nsys,
     const double *__restrict__ per_atom, const int32_t *__restrict__ types,
-    double4 *__restrict__ spos, int4 *__restrict__ sinfo, int32_t *__restrict__ stype) {
+    double4 *__restrict__ spos, int4 *__restrict__ sinfo, int32_t *__restrict__ stype,
+    int32_t *__restrict__ scount) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nat) return;
     const int4 rc = raw4[a];
@@ -265,6 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
     spos[s] = make_double4(p[0], p[1], p[2], per_atom ? per_atom[a] : 0.0);
     sinfo[s] = make_int4(base[0], base[1], base[2], a - g.atom_base);
     if (stype) stype[s] = types ? types[a] : 0;
+    if (scount) scount[s] = 0; /* the half count accumulates here */
 }
 
 /* ---------------------------------------------------------------- scan */
@@ -584,7 +586,7 @@ struct Cand {
     double pw;             /* radius (per-atom cutoffs) */
     double o0, o1, o2;     /* cell offset vector off = (z b3 + y b2) + x b1 */
     int tj;                /* type (per-type cutoffs) */
-    int j;                 /* atom index (system-local), fill only */
+    int j;                 /* fill: atom index (system-local); count: cell-sorted index */
     int c0, c1, c2;        /* shift numerator part x - raw_j (fill only) */
 };
 
@@ -615,6 +617,8 @@ __device__ __forceinline__ void load_candidate(const PairArgs &A, const HomeCell
         C.c0 = x - bj.x;
         C.c1 = y - bj.y;
         C.c2 = z - bj.z;
+    } else {
+        C.j = sj;
     }
 }
 
@@ -694,6 +698,69 @@ __device__ __forceinline__ void count_cell(const PairArgs &A, int h, SegLds &S,
     }
 }
 
+/*
+ * Count pass over half the neighbour-cell offsets. The test is symmetric:
+ * for the reversed pair every operand is negated exactly (pi - pj = -(pj -
+ * pi), the offset vector of -x is -off), so dr and r2 carry the same bits
+ * and (j, i, -x) is accepted exactly when (i, j, x) is; the per-atom sum of
+ * radii commutes, and per-type tables are used here only when symmetric.
+ * The wave therefore visits the offsets from (0,0,0) on in traversal order
+ * - (z, y, x) lexicographically >= 0, the segments from the centre to the
+ * end - and inside the home cell only candidates after the home atom. Each
+ * accepted pair counts for both atoms: the home atom's count is the ballot's
+ * popcount, the candidate's the number of home atoms it passed; both are
+ * added atomically into the cell-sorted counts.
+ */
+template <int CUT, bool HASHED>
+__device__ __forceinline__ void count_cell_half(const PairArgs &A, int h, SegLds &S,
+                                                lds_dptr pt_sq) {
+    HomeCell H;
+    if (!load_home<HASHED>(A, h, H)) return;
+    const int lane = lane_id();
+    for (int grp0 = 0; grp0 < H.m; grp0 += kWave) {
+        const int mg = uniform(min(kWave, H.m - grp0));
+        int acc = 0;
+        for (int sg = H.centre; sg < H.nseg; sg += kWave) {
+            const int nsg = min(kWave, H.nseg - sg);
+            int self_start;
+            const int total = build_segments<HASHED>(A, H, sg, nsg, lane, S, &self_start);
+            for (int cb = 0; cb < total; cb += kWave) {
+                Cand C;
+                load_candidate<CUT, false>(A, H, S, nsg, cb + lane, total, C);
+                const int c = cb + lane;
+                /* the centre segment leads the first group: candidate c < m
+                   is home atom c, taken by home atom grp0 + t only if after it */
+                const int lim = (sg == H.centre && c < H.m) ? c - grp0 : kWave;
+                uint32_t part = 0;
+                int cj = 0;
+                for (int t = 0; t < mg; t++) {
+                    const int si = H.hb + grp0 + t;
+                    const double4 pi = load_const(A.spos, si);
+                    double d0, d1, d2;
+                    const bool ok =
+                        pair_test<CUT>(A, H, C, si, pi, d0, d1, d2, pt_sq) & (t < lim);
+                    const uint64_t mask = __builtin_amdgcn_ballot_w64(ok);
+                    cj += ok ? 1 : 0;
+                    part = write_lane(part, static_cast<uint32_t>(__popcll(mask)), t);
+                }
+                acc += static_cast<int>(part);
+                if (cj) atomicAdd(A.scount + C.j, cj);
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (lane < mg && acc) atomicAdd(A.scount + H.hb + grp0 + lane, acc);
+    }
+}
+
+/* Half count -> counts by global atom index. A system's atoms occupy the
+   same index range in cell-sorted and in original order. */
+__global__ __launch_bounds__(kBlock) void k_unsort_counts(PairArgs A) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= A.nat) return;
+    const int sys = system_of(A.atom_base, A.nsys, s);
+    A.counts[A.atom_base[sys] + A.sinfo[s].w] = A.scount[s];
+}
+
 /* Per-type cutoff tables up to this many entries are staged in LDS once per
    block (the test reads one entry per candidate and home atom); larger ones
    are read from global memory. */
@@ -708,7 +775,7 @@ __device__ __forceinline__ lds_dptr stage_types(const PairArgs &A, double *s_pt)
     return (lds_dptr)(s_pt);
 }
 
-template <int CUT, bool HASHED>
+template <int CUT, bool HASHED, bool HALF>
 __global__ __launch_bounds__(kBlock) void k_count(PairArgs A) {
     __shared__ SegLds s_lds[kWavesPerBlock];
     __shared__ double s_pt[CUT == CUT_PER_TYPE_LDS ? kLdsTypes : 1];
@@ -717,7 +784,10 @@ __global__ __launch_bounds__(kBlock) void k_count(PairArgs A) {
     const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
     const int h = blk * kWavesPerBlock + wib;
     if (h >= A.home_cells) return;
-    count_cell<CUT, HASHED>(A, h, s_lds[wib], pt_sq);
+    if (HALF)
+        count_cell_half<CUT, HASHED>(A, h, s_lds[wib], pt_sq);
+    else
+        count_cell<CUT, HASHED>(A, h, s_lds[wib], pt_sq);
 }
 
 /* ------------------------------------------------------------- fill pass */
@@ -1113,11 +1183,13 @@ hipError_t launch_rank_gather(const double *r, const int4 *raw4, const int32_t *
                               const int32_t *cell_count, const SysGeom *geo,
                               const int32_t *atom_base, int32_t nsys,
                               const double *per_atom, const int32_t *types, double4 *spos,
-                              int4 *sinfo, int32_t *stype, hipStream_t stream) {
+                              int4 *sinfo, int32_t *stype, int32_t *scount,
+                              hipStream_t stream) {
     if (nat <= 0) return hipSuccess;
     k_rank_gather<<<grid_for(nat), kBlock, 0, stream>>>(r, raw4, tmp, nat, cell_first,
                                                         cell_count, geo, atom_base, nsys,
-                                                        per_atom, types, spos, sinfo, stype);
+                                                        per_atom, types, spos, sinfo, stype,
+                                                        scount);
     return hipGetLastError();
 }
 
@@ -1162,7 +1234,12 @@ hipError_t launch_scan_i64_to_i64(const int64_t *in, int64_t n, int64_t *out,
 
 template <int CUT, bool HASHED>
 static void launch_count_t(const PairArgs &a, unsigned nblocks, hipStream_t stream) {
-    k_count<CUT, HASHED><<<nblocks, kBlock, 0, stream>>>(a);
+    if (a.half) {
+        k_count<CUT, HASHED, true><<<nblocks, kBlock, 0, stream>>>(a);
+        if (a.nat > 0) k_unsort_counts<<<grid_for(a.nat), kBlock, 0, stream>>>(a);
+    } else {
+        k_count<CUT, HASHED, false><<<nblocks, kBlock, 0, stream>>>(a);
+    }
 }
 
 template <int MODE, int CUT, bool HASHED>

@@ -98,6 +98,23 @@ def test_per_type_out_of_range_types_keep_global_test(gpu):
                                                [1.2, 0.9, 1.7]]), numbers=types)
 
 
+@pytest.mark.parametrize("L", [8.0, 2.5])
+def test_per_type_asymmetric_table(gpu, L):
+    """A per-type matrix with cut[a][b] != cut[b][a]: the pair (i, j) uses
+    the entry [type i][type j] (neighbour_visit.hh:100-108), so (i, j) and
+    (j, i) can differ and the count must visit every offset. Also in a box
+    smaller than the cutoff (multi-image shifts)."""
+    rng = np.random.default_rng(5)
+    n = 300 if L > 4 else 12
+    pos = rng.uniform(0, L, (n, 3))
+    types = rng.integers(-1, 3, n).astype(np.int32)
+    cut = np.array([[1.0, 1.6, 0.7], [0.9, 2.0, 1.4], [1.8, 0.6, 1.7]])
+    ref = _check(pos, L * np.eye(3), True, cut, numbers=types, backends=(0, 1))
+    i, j = ref["i"], ref["j"]
+    pairs = set(zip(i.tolist(), j.tolist()))
+    assert any((b, a) not in pairs for a, b in pairs), "table asymmetry not exercised"
+
+
 # ------------------------------------------------------------ reference scenarios
 
 @pytest.mark.parametrize("pbc", [True, False, (True, False, True), (False, True, False)])
