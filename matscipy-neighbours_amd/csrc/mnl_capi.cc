@@ -227,17 +227,17 @@ int ensure_workspaces(mnl_plan *p, hipStream_t stream, int32_t nat, int64_t cell
 
 /*
  * Binning of a build whose geometry (geo, atom_base, cell_base) is already
- * on the device: clear -> bin -> scan -> scatter -> rank/gather. Records no
- * memset and does not synchronise (capturable).
+ * on the device: clear (unless `cleared`) -> bin -> scan -> scatter ->
+ * rank/gather. Records no memset and does not synchronise (capturable).
  */
 int enqueue_binning(mnl_plan *p, hipStream_t stream, const double *positions,
-                    const double *per_atom, const int32_t *types) {
+                    const double *per_atom, const int32_t *types, bool cleared = false) {
     const bool hashed = p->backend == MNL_BACKEND_HASHED;
     const int64_t cells = p->home_cells;
     const int32_t nat = p->nat;
-    MNL_HIP(launch_fill_i32(p->cell_count.as<int32_t>(), cells, 0, stream));
+    if (!cleared) MNL_HIP(launch_fill_i32(p->cell_count.as<int32_t>(), cells, 0, stream));
     if (hashed) {
-        MNL_HIP(launch_fill_i64(p->hkey.as<int64_t>(), cells, -1, stream));
+        if (!cleared) MNL_HIP(launch_fill_i64(p->hkey.as<int64_t>(), cells, -1, stream));
         MNL_HIP(launch_bin_hashed(positions, nat, p->geo.as<SysGeom>(), p->raw4.as<int4>(),
                                   p->ticket.as<int32_t>(), p->hkey.as<int64_t>(), p->hmask,
                                   p->cell_count.as<int32_t>(), stream));
@@ -262,19 +262,22 @@ int enqueue_binning(mnl_plan *p, hipStream_t stream, const double *positions,
 }
 
 /*
- * binning -> count -> scan. Leaves the pair total in p->total (device);
- * does not synchronise and records no memset, so it can be captured in a
- * graph.
+ * binning -> count -> scan. Leaves the pair total in p->total (device) and,
+ * if given, in *total_copy (device); does not synchronise and records no
+ * memset, so it can be captured in a graph.
  */
 int enqueue_build(mnl_plan *p, hipStream_t stream, const double *positions,
-                  const double *per_atom, const int32_t *types) {
-    if (enqueue_binning(p, stream, positions, per_atom, types) != MNL_SUCCESS) return MNL_ERROR;
+                  const double *per_atom, const int32_t *types,
+                  int64_t *total_copy = nullptr, bool cleared = false) {
+    if (enqueue_binning(p, stream, positions, per_atom, types, cleared) != MNL_SUCCESS)
+        return MNL_ERROR;
     const bool hashed = p->backend == MNL_BACKEND_HASHED;
     const int32_t nat = p->nat;
     PairArgs a = pair_args(p);
     MNL_HIP(launch_pairs(0, p->cut, hashed, a, stream));
     MNL_HIP(launch_scan_i32_to_i64(p->counts.as<int32_t>(), nat, p->offsets.as<int64_t>(),
-                                   p->partial.as<int64_t>(), p->total.as<int64_t>(), stream));
+                                   p->partial.as<int64_t>(), p->total.as<int64_t>(), stream,
+                                   total_copy));
     return MNL_SUCCESS;
 }
 
@@ -286,7 +289,8 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
                   const double cell[9], const double inv_cell[9], const int32_t pbc[3],
                   int32_t nat, const double *positions, double cutoff,
                   const double *per_atom, const double *per_type_sq, int32_t ncutoffs,
-                  const int32_t *types, int backend, bool with_count = true) {
+                  const int32_t *types, int backend, bool with_count = true,
+                  int64_t *total_copy = nullptr) {
     p->counted = false;
     p->batch_ready = false;
     if (nat < 0) return fail("negative atom count");
@@ -334,11 +338,13 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
         MNL_SUCCESS)
         return MNL_ERROR;
 
-    /* parameters are written on the device by a one-thread kernel: no
-       host-to-device copy, nothing read from host memory later */
+    /* parameters are written on the device by the kernel that clears the
+       grid: no host-to-device copy, nothing read from host memory later */
     MNL_HIP(launch_set_params(p->geo.as<SysGeom>(), g, p->atom_base.as<int32_t>(),
                               p->cell_base.as<int32_t>(), nat,
-                              static_cast<int32_t>(hashed ? 0 : ncells), stream));
+                              static_cast<int32_t>(hashed ? 0 : ncells),
+                              p->cell_count.as<int32_t>(), cells_or_slots,
+                              hashed ? p->hkey.as<int64_t>() : nullptr, stream));
     if (p->cut == CUT_PER_TYPE && ncutoffs > 0) {
         /* the table is uploaded only when it differs from the plan's copy, so
            steady-state builds with fixed cutoffs stay asynchronous */
@@ -362,8 +368,8 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
             MNL_HIP(hipStreamSynchronize(stream));
         }
     }
-    if (!with_count) return enqueue_binning(p, stream, positions, per_atom, types);
-    return enqueue_build(p, stream, positions, per_atom, types);
+    if (!with_count) return enqueue_binning(p, stream, positions, per_atom, types, true);
+    return enqueue_build(p, stream, positions, per_atom, types, total_copy, true);
 }
 
 int read_total(mnl_plan *p, hipStream_t stream, int64_t *npairs) {
@@ -652,12 +658,9 @@ int mnl_plan_count_async(mnl_plan *p, void *stream, const double origin[3],
     if (!ds.ok) return fail("cannot switch to the plan's device");
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (enqueue_count(p, s, origin, cell, inv_cell, pbc, nat, positions, cutoff,
-                      per_atom_cutoff, per_type_cutoff_sq, ncutoffs, types,
-                      backend) != MNL_SUCCESS)
+                      per_atom_cutoff, per_type_cutoff_sq, ncutoffs, types, backend, true,
+                      npairs_device) != MNL_SUCCESS)
         return MNL_ERROR;
-    if (npairs_device)
-        MNL_HIP(hipMemcpyAsync(npairs_device, p->total.ptr, sizeof(int64_t),
-                               hipMemcpyDeviceToDevice, s));
     p->counted = true;
     p->npairs = -1; /* unknown on the host until read */
     return MNL_SUCCESS;
@@ -947,10 +950,8 @@ int batch_count_enqueue(mnl_plan *p, hipStream_t st, const double *positions,
                     "single-system count on the plan replaces it)");
     if (p->nat > 0 && !positions) return fail("positions is NULL");
     p->counted = false;
-    if (enqueue_build(p, st, positions, nullptr, nullptr) != MNL_SUCCESS) return MNL_ERROR;
-    if (npairs_device)
-        MNL_HIP(hipMemcpyAsync(npairs_device, p->total.ptr, sizeof(int64_t),
-                               hipMemcpyDeviceToDevice, st));
+    if (enqueue_build(p, st, positions, nullptr, nullptr, npairs_device) != MNL_SUCCESS)
+        return MNL_ERROR;
     p->counted = true;
     p->npairs = -1;
     return MNL_SUCCESS;

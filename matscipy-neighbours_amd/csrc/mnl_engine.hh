@@ -111,15 +111,15 @@ hipError_t launch_rank_gather(const double *r, const int4 *raw4, const int32_t *
                               double4 *spos, int4 *sinfo, int32_t *stype,
                               int32_t *scount, hipStream_t stream);
 /* Exclusive scan of n int32 values; out gets n (+1 with the total) entries.
-   partial needs scan_partial_size(n) int64 entries. total (device, may be
-   NULL) receives the sum. */
+   partial needs scan_partial_size(n) int64 entries. total and total2
+   (device, may be NULL) receive the sum. */
 int64_t scan_partial_size(int64_t n);
 hipError_t launch_scan_i32_to_i32(const int32_t *in, int64_t n, int32_t *out,
                                   int64_t *partial, int64_t *total,
                                   hipStream_t stream);
 hipError_t launch_scan_i32_to_i64(const int32_t *in, int64_t n, int64_t *out,
                                   int64_t *partial, int64_t *total,
-                                  hipStream_t stream);
+                                  hipStream_t stream, int64_t *total2 = nullptr);
 hipError_t launch_scan_i64_to_i64(const int64_t *in, int64_t n, int64_t *out,
                                   int64_t *partial, int64_t *total,
                                   hipStream_t stream);
@@ -142,8 +142,11 @@ hipError_t launch_triplet_fill(const int32_t *first_i, int32_t nrows, const int3
    out_j = neighbour index, out_D = D, counts = row counts; no count pass). */
 hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
                         hipStream_t stream);
+/* One-system parameters into geo/atom_base/cell_base, and the grid cleared:
+   cell_count[0..nslots) = 0 and, if hkey is given, hkey[0..nslots) = -1. */
 hipError_t launch_set_params(SysGeom *geo, const SysGeom &g, int32_t *atom_base,
                              int32_t *cell_base, int32_t nat, int32_t ncells,
+                             int32_t *cell_count, int64_t nslots, int64_t *hkey,
                              hipStream_t stream);
 hipError_t launch_sqrt_probe(const double *x, double *y, int64_t n,
                              hipStream_t stream);

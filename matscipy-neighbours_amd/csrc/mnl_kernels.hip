@@ -155,21 +155,49 @@ __device__ __forceinline__ void raw_cell_of(const double *__restrict__ r, int a,
     }
 }
 
+/* Histogram ticket with one atomic per distinct counter in the wave: the
+   lanes sharing a counter are grouped (one ballot per distinct value), the
+   group's lowest lane adds the group size, and each lane's ticket is that
+   base plus its position in the group. Called by every lane of the wave;
+   lanes with valid == false get no ticket and add nothing. */
+__device__ __forceinline__ int wave_ticket(int32_t *__restrict__ counters, int key, bool valid) {
+    const int lane = lane_id();
+    uint64_t todo = __builtin_amdgcn_ballot_w64(valid);
+    uint64_t grp = 0;
+    while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const int lk = __builtin_amdgcn_readlane(key, leader);
+        const uint64_t same = __builtin_amdgcn_ballot_w64(valid && key == lk) & todo;
+        if ((same >> lane) & 1) grp = same;
+        todo &= ~same;
+    }
+    const int first = valid ? __builtin_ctzll(grp) : lane;
+    int base = 0;
+    if (valid && lane == first) base = atomicAdd(&counters[key], __popcll(grp));
+    base = __shfl(base, first, kWave);
+    return base + __popcll(grp & ((1ull << lane) - 1));
+}
+
 __global__ __launch_bounds__(kBlock) void k_bin_dense(
     const double *__restrict__ r, int32_t nat, const SysGeom *__restrict__ geo,
     const int32_t *__restrict__ atom_base, int32_t nsys, int4 *__restrict__ raw4,
     int32_t *__restrict__ ticket, int32_t *__restrict__ cell_count) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nat) return;
-    const SysGeom &g = geo[system_of(atom_base, nsys, a)];
-    int raw[3];
-    raw_cell_of(r, a, g, raw);
-    int c[3];
+    const bool valid = a < nat;
+    int raw[3] = {0, 0, 0};
+    int cell = -1;
+    if (valid) {
+        const SysGeom &g = geo[system_of(atom_base, nsys, a)];
+        raw_cell_of(r, a, g, raw);
+        int c[3];
 #pragma unroll
-    for (int k = 0; k < 3; k++)
-        c[k] = g.pbc[k] ? wrap_idx(raw[k], g.n[k]) : clamp_idx(raw[k], g.n[k]);
-    const int cell = g.cell_base + c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
-    ticket[a] = atomicAdd(&cell_count[cell], 1);
+        for (int k = 0; k < 3; k++)
+            c[k] = g.pbc[k] ? wrap_idx(raw[k], g.n[k]) : clamp_idx(raw[k], g.n[k]);
+        cell = g.cell_base + c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
+    }
+    const int tk = wave_ticket(cell_count, cell, valid);
+    if (!valid) return;
+    ticket[a] = tk;
     raw4[a] = make_int4(raw[0], raw[1], raw[2], cell);
 }
 
@@ -181,27 +209,34 @@ __global__ __launch_bounds__(kBlock) void k_bin_hashed(
     int4 *__restrict__ raw4, int32_t *__restrict__ ticket, int64_t *__restrict__ hkey,
     int64_t hmask, int32_t *__restrict__ hcount) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nat) return;
-    const SysGeom &g = geo[0];
-    int raw[3];
-    raw_cell_of(r, a, g, raw);
-    int c[3];
+    const bool valid = a < nat;
+    int raw[3] = {0, 0, 0};
+    int64_t h = -1;
+    if (valid) {
+        const SysGeom &g = geo[0];
+        raw_cell_of(r, a, g, raw);
+        int c[3];
 #pragma unroll
-    for (int k = 0; k < 3; k++)
-        c[k] = g.pbc[k] ? wrap_idx(raw[k], g.n[k]) : clamp_idx(raw[k], g.n[k]);
-    const int64_t key = static_cast<int64_t>(c[0]) +
-                        static_cast<int64_t>(g.n[0]) *
-                            (static_cast<int64_t>(c[1]) + static_cast<int64_t>(g.n[1]) * c[2]);
-    int64_t h = cell_hash(key) & hmask;
-    auto *slots = reinterpret_cast<unsigned long long *>(hkey);
-    const unsigned long long empty = ~0ull;
-    while (true) {
-        const unsigned long long old =
-            atomicCAS(&slots[h], empty, static_cast<unsigned long long>(key));
-        if (old == empty || static_cast<int64_t>(old) == key) break;
-        h = (h + 1) & hmask;
+        for (int k = 0; k < 3; k++)
+            c[k] = g.pbc[k] ? wrap_idx(raw[k], g.n[k]) : clamp_idx(raw[k], g.n[k]);
+        const int64_t key =
+            static_cast<int64_t>(c[0]) +
+            static_cast<int64_t>(g.n[0]) *
+                (static_cast<int64_t>(c[1]) + static_cast<int64_t>(g.n[1]) * c[2]);
+        h = cell_hash(key) & hmask;
+        auto *slots = reinterpret_cast<unsigned long long *>(hkey);
+        const unsigned long long empty = ~0ull;
+        /* ends: at most nat keys in >= 2 nat + 1 slots */
+        while (true) {
+            const unsigned long long old =
+                atomicCAS(&slots[h], empty, static_cast<unsigned long long>(key));
+            if (old == empty || static_cast<int64_t>(old) == key) break;
+            h = (h + 1) & hmask;
+        }
     }
-    ticket[a] = atomicAdd(&hcount[h], 1);
+    const int tk = wave_ticket(hcount, static_cast<int>(h), valid);
+    if (!valid) return;
+    ticket[a] = tk;
     raw4[a] = make_int4(raw[0], raw[1], raw[2], static_cast<int>(h));
 }
 
@@ -295,7 +330,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const TI *__restrict__ i
 
 /* Single block: exclusive scan of the nb block sums in place; total out. */
 __global__ __launch_bounds__(1024) void k_scan_spine(int64_t *__restrict__ partial, int64_t nb,
-                                                     int64_t *__restrict__ total) {
+                                                     int64_t *__restrict__ total,
+                                                     int64_t *__restrict__ total2) {
     __shared__ int64_t s_w[1024 / kWave];
     __shared__ int64_t s_carry;
     if (threadIdx.x == 0) s_carry = 0;
@@ -314,6 +350,7 @@ __global__ __launch_bounds__(1024) void k_scan_spine(int64_t *__restrict__ parti
         __syncthreads();
     }
     if (threadIdx.x == 0 && total) *total = s_carry;
+    if (threadIdx.x == 0 && total2) *total2 = s_carry;
 }
 
 template <typename TI, typename TO>
@@ -516,9 +553,10 @@ __device__ __forceinline__ bool load_home(const PairArgs &A, int h, HomeCell &H)
 /* Segment table of one group of up to 64 neighbour-cell offsets, in the
    reference's traversal order. */
 struct SegLds {
-    int st[kWave];    /* first candidate index of each segment */
-    int sb[kWave];    /* first cell-sorted atom of each segment */
-    int sxyz[kWave];  /* packed cell offset (x, y, z) */
+    double so[3][kWave]; /* offset vector off = (z b3 + y b2) + x b1 */
+    int st[kWave];       /* first candidate index of each segment */
+    int sb[kWave];       /* first cell-sorted atom of each segment */
+    int sxyz[kWave];     /* packed cell offset (x, y, z) */
 };
 
 /* Builds the table of segments sg .. sg+nsg-1; returns the group's candidate
@@ -555,6 +593,10 @@ __device__ __forceinline__ int build_segments(const PairArgs &A, const HomeCell 
                 dense_slice(A, H.cbase, H.n0, H.n1, cj0, cj1, cj2, begin, len);
         }
         pxyz = pack3(x, y, z);
+        /* the offset vector, neighbour_visit.hh:67,73-74,80-81 */
+        S.so[0][lane] = (z * H.b20 + y * H.b10) + x * H.b00;
+        S.so[1][lane] = (z * H.b21 + y * H.b11) + x * H.b01;
+        S.so[2][lane] = (z * H.b22 + y * H.b12) + x * H.b02;
     }
     const int incl = wave_incl_scan(len);
     const int total = uniform(__shfl(incl, kWave - 1, kWave));
@@ -591,8 +633,8 @@ struct Cand {
 };
 
 /* Candidate c of the group's candidate space: its segment (the last one
-   starting at or before c; empty segments never win), cell-sorted index and
-   record. The offset vector follows neighbour_visit.hh:67,73-74,80-81. */
+   starting at or before c; empty segments never win), cell-sorted index,
+   record and the segment's offset vector. */
 template <int CUT, bool FILL>
 __device__ __forceinline__ void load_candidate(const PairArgs &A, const HomeCell &H,
                                                const SegLds &S, int nsg, int c, int total,
@@ -600,11 +642,9 @@ __device__ __forceinline__ void load_candidate(const PairArgs &A, const HomeCell
     const bool cv = c < total;
     const int p = last_at_or_below(S.st, nsg, c);
     const int sj = cv ? S.sb[p] + (c - S.st[p]) : H.hb;
-    int x, y, z;
-    unpack3(S.sxyz[p], x, y, z);
-    C.o0 = (z * H.b20 + y * H.b10) + x * H.b00;
-    C.o1 = (z * H.b21 + y * H.b11) + x * H.b01;
-    C.o2 = (z * H.b22 + y * H.b12) + x * H.b02;
+    C.o0 = S.so[0][p];
+    C.o1 = S.so[1][p];
+    C.o2 = S.so[2][p];
     const double4 pj = A.spos[sj];
     C.px = cv ? pj.x : __builtin_nan("");
     C.py = pj.y;
@@ -612,6 +652,8 @@ __device__ __forceinline__ void load_candidate(const PairArgs &A, const HomeCell
     C.pw = pj.w;
     C.tj = (CUT == CUT_PER_TYPE || CUT == CUT_PER_TYPE_LDS) ? A.stype[sj] : 0;
     if (FILL) {
+        int x, y, z;
+        unpack3(S.sxyz[p], x, y, z);
         const int4 bj = A.sinfo[sj];
         C.j = bj.w;
         C.c0 = x - bj.x;
@@ -711,6 +753,42 @@ __device__ __forceinline__ void count_cell(const PairArgs &A, int h, SegLds &S,
  * popcount, the candidate's the number of home atoms it passed; both are
  * added atomically into the cell-sorted counts.
  */
+/* acc + (bit `lane` of mask): one add with the mask as carry-in. */
+__device__ __forceinline__ int add_lane_bit(int acc, uint64_t mask) {
+    uint64_t carry_out;
+    asm volatile("v_addc_co_u32_e64 %0, %1, 0, %0, %2"
+                 : "+v"(acc), "=s"(carry_out)
+                 : "s"(mask));
+    return acc;
+}
+
+/* One candidate chunk against home atoms grp0 .. grp0+mg-1 (half count):
+   lane t of `part` gets home atom t's accepted candidates, cj the home
+   atoms each candidate accepted. CENTRE: the chunk holds home-cell atoms
+   (candidates c < m); home atom g takes only those with c > g, a scalar
+   mask over the chunk's low lanes. */
+template <int CUT, bool CENTRE>
+__device__ __forceinline__ void half_sweep(const PairArgs &A, const HomeCell &H, const Cand &C,
+                                           int grp0, int mg, int cb, lds_dptr pt_sq,
+                                           uint32_t &part, int &cj) {
+    for (int t = 0; t < mg; t++) {
+        const int si = H.hb + grp0 + t;
+        const double4 pi = load_const(A.spos, si);
+        double d0, d1, d2;
+        uint64_t mask =
+            __builtin_amdgcn_ballot_w64(pair_test<CUT>(A, H, C, si, pi, d0, d1, d2, pt_sq));
+        if (CENTRE) {
+            const int hi = min(grp0 + t, H.m - 1) - cb; /* last lane to drop */
+            if (hi >= kWave - 1)
+                mask = 0;
+            else if (hi >= 0)
+                mask &= ~0ull << (hi + 1);
+        }
+        cj = add_lane_bit(cj, mask);
+        part = write_lane(part, static_cast<uint32_t>(__popcll(mask)), t);
+    }
+}
+
 template <int CUT, bool HASHED>
 __device__ __forceinline__ void count_cell_half(const PairArgs &A, int h, SegLds &S,
                                                 lds_dptr pt_sq) {
@@ -727,22 +805,14 @@ __device__ __forceinline__ void count_cell_half(const PairArgs &A, int h, SegLds
             for (int cb = 0; cb < total; cb += kWave) {
                 Cand C;
                 load_candidate<CUT, false>(A, H, S, nsg, cb + lane, total, C);
-                const int c = cb + lane;
-                /* the centre segment leads the first group: candidate c < m
-                   is home atom c, taken by home atom grp0 + t only if after it */
-                const int lim = (sg == H.centre && c < H.m) ? c - grp0 : kWave;
                 uint32_t part = 0;
                 int cj = 0;
-                for (int t = 0; t < mg; t++) {
-                    const int si = H.hb + grp0 + t;
-                    const double4 pi = load_const(A.spos, si);
-                    double d0, d1, d2;
-                    const bool ok =
-                        pair_test<CUT>(A, H, C, si, pi, d0, d1, d2, pt_sq) & (t < lim);
-                    const uint64_t mask = __builtin_amdgcn_ballot_w64(ok);
-                    cj += ok ? 1 : 0;
-                    part = write_lane(part, static_cast<uint32_t>(__popcll(mask)), t);
-                }
+                /* the centre segment leads the first group: its candidates
+                   c < m are the home atoms, taken only after the home atom */
+                if (sg == H.centre && cb < H.m)
+                    half_sweep<CUT, true>(A, H, C, grp0, mg, cb, pt_sq, part, cj);
+                else
+                    half_sweep<CUT, false>(A, H, C, grp0, mg, cb, pt_sq, part, cj);
                 acc += static_cast<int>(part);
                 if (cj) atomicAdd(A.scount + C.j, cj);
             }
@@ -994,9 +1064,18 @@ __global__ __launch_bounds__(kBlock) void k_fill(PairArgs A) {
 
 /* Device-side parameter set-up (one thread): keeps the whole build free of
    host-to-device copies, so that it can be captured in a graph. */
+/* One-system parameters, and the cleared grid: cell (or slot) counts to 0,
+   hash keys (if any) to empty. */
 __global__ void k_set_params(SysGeom *__restrict__ geo, SysGeom g, int32_t *__restrict__ atom_base,
-                             int32_t *__restrict__ cell_base, int32_t nat, int32_t ncells) {
-    if (threadIdx.x != 0) return;
+                             int32_t *__restrict__ cell_base, int32_t nat, int32_t ncells,
+                             int32_t *__restrict__ cell_count, int64_t nslots,
+                             int64_t *__restrict__ hkey) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < nslots) {
+        cell_count[t] = 0;
+        if (hkey) hkey[t] = -1;
+    }
+    if (t != 0) return;
     geo[0] = g;
     atom_base[0] = 0;
     atom_base[1] = nat;
@@ -1197,7 +1276,8 @@ int64_t scan_partial_size(int64_t n) { return (n + kScanTile - 1) / kScanTile + 
 
 template <typename TI, typename TO>
 static hipError_t scan_impl(const TI *in, int64_t n, TO *out, int64_t *partial,
-                            int64_t *total, bool write_total, hipStream_t stream) {
+                            int64_t *total, int64_t *total2, bool write_total,
+                            hipStream_t stream) {
     if (n <= 0) {
         if (write_total) {
             /* out[0] = 0 */
@@ -1207,11 +1287,12 @@ static hipError_t scan_impl(const TI *in, int64_t n, TO *out, int64_t *partial,
                 k_fill_i32<<<1, kBlock, 0, stream>>>(reinterpret_cast<int32_t *>(out), 1, 0);
         }
         if (total) k_fill_i64<<<1, kBlock, 0, stream>>>(total, 1, 0);
+        if (total2) k_fill_i64<<<1, kBlock, 0, stream>>>(total2, 1, 0);
         return hipGetLastError();
     }
     const int64_t nb = (n + kScanTile - 1) / kScanTile;
     k_scan_reduce<TI><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(in, n, partial);
-    k_scan_spine<<<1, 1024, 0, stream>>>(partial, nb, total);
+    k_scan_spine<<<1, 1024, 0, stream>>>(partial, nb, total, total2);
     k_scan_apply<TI, TO><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(
         in, n, partial, out, write_total ? 1 : 0);
     return hipGetLastError();
@@ -1219,17 +1300,18 @@ static hipError_t scan_impl(const TI *in, int64_t n, TO *out, int64_t *partial,
 
 hipError_t launch_scan_i32_to_i32(const int32_t *in, int64_t n, int32_t *out,
                                   int64_t *partial, int64_t *total, hipStream_t stream) {
-    return scan_impl<int32_t, int32_t>(in, n, out, partial, total, false, stream);
+    return scan_impl<int32_t, int32_t>(in, n, out, partial, total, nullptr, false, stream);
 }
 
 hipError_t launch_scan_i32_to_i64(const int32_t *in, int64_t n, int64_t *out,
-                                  int64_t *partial, int64_t *total, hipStream_t stream) {
-    return scan_impl<int32_t, int64_t>(in, n, out, partial, total, true, stream);
+                                  int64_t *partial, int64_t *total, hipStream_t stream,
+                                  int64_t *total2) {
+    return scan_impl<int32_t, int64_t>(in, n, out, partial, total, total2, true, stream);
 }
 
 hipError_t launch_scan_i64_to_i64(const int64_t *in, int64_t n, int64_t *out,
                                   int64_t *partial, int64_t *total, hipStream_t stream) {
-    return scan_impl<int64_t, int64_t>(in, n, out, partial, total, true, stream);
+    return scan_impl<int64_t, int64_t>(in, n, out, partial, total, nullptr, true, stream);
 }
 
 template <int CUT, bool HASHED>
@@ -1294,8 +1376,10 @@ hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
 
 hipError_t launch_set_params(SysGeom *geo, const SysGeom &g, int32_t *atom_base,
                              int32_t *cell_base, int32_t nat, int32_t ncells,
+                             int32_t *cell_count, int64_t nslots, int64_t *hkey,
                              hipStream_t stream) {
-    k_set_params<<<1, kWave, 0, stream>>>(geo, g, atom_base, cell_base, nat, ncells);
+    k_set_params<<<grid_for(std::max<int64_t>(nslots, 1)), kBlock, 0, stream>>>(
+        geo, g, atom_base, cell_base, nat, ncells, cell_count, nslots, hkey);
     return hipGetLastError();
 }
 
