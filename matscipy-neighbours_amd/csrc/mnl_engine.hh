@@ -41,8 +41,16 @@ struct SysGeom {
     int32_t cell_base;  /* first dense cell of the system */
     int32_t ncells;
     int32_t nseg;       /* (2nx+1)(2ny+1)(2nz+1) neighbour-cell offsets */
+    /* division by n[0], n[1], 2nx+1, 2ny+1 as a multiply and shift:
+       x / d == (x * dmul) >> dshift for 0 <= x < 2^31 (make_div_magic) */
+    uint32_t dmul[4];
+    int32_t dshift[4];
     int64_t pad;
 };
+
+/* Multiplier and shift with x / d == (uint64(x) * mul) >> shift for all
+   0 <= x < 2^31 (d >= 1). */
+void make_div_magic(int32_t d, uint32_t &mul, int32_t &shift);
 
 /* Host geometry (src/libneighbours/cell_list.cc:153-182 and the search range
    of neighbour_list.cc:176-178), with the reference's operation order.

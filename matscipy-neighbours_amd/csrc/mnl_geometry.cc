@@ -74,8 +74,26 @@ bool make_sys_geom(const double origin[3], const double cell[9],
     const int64_t nc = static_cast<int64_t>(g.n[0]) * g.n[1] * g.n[2];
     g.ncells = nc > INT32_MAX ? INT32_MAX : static_cast<int32_t>(nc);
     g.nseg = (2 * g.nxyz[0] + 1) * (2 * g.nxyz[1] + 1) * (2 * g.nxyz[2] + 1);
+    const int32_t divisors[4] = {g.n[0], g.n[1], 2 * g.nxyz[0] + 1, 2 * g.nxyz[1] + 1};
+    for (int k = 0; k < 4; k++) make_div_magic(divisors[k], g.dmul[k], g.dshift[k]);
     g.pad = 0;
     return true;
+}
+
+/* s = ceil(log2 d), mul = floor(2^(31+s) / d) + 1 < 2^32. For x < 2^31,
+   x * mul / 2^(31+s) = x/d + x*e/2^(31+s) with 0 < e <= 1, and the excess
+   is below 2^-s <= 1/d, which the fractional part of x/d (at most
+   1 - 1/d) cannot carry past the next integer. d = 1: mul 1, shift 0. */
+void make_div_magic(int32_t d, uint32_t &mul, int32_t &shift) {
+    if (d <= 1) {
+        mul = 1;
+        shift = 0;
+        return;
+    }
+    int s = 0;
+    while ((int64_t(1) << s) < d) s++;
+    mul = static_cast<uint32_t>((uint64_t(1) << (31 + s)) / uint64_t(d) + 1);
+    shift = 31 + s;
 }
 
 }  // namespace mnl

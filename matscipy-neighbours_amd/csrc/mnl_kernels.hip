@@ -62,6 +62,22 @@ __device__ __forceinline__ int floor_div(int a, int n) {
     return (a % n != 0 && a < 0) ? q - 1 : q;
 }
 
+/* x / d for 0 <= x < 2^31 with the multiplier and shift of make_div_magic
+   (scalar instructions when x is wave-uniform). */
+__device__ __forceinline__ int div_magic(int x, uint32_t mul, int shift) {
+    return static_cast<int>((static_cast<uint64_t>(static_cast<uint32_t>(x)) * mul) >> shift);
+}
+
+/* Periodic image of cell index cj, in [0, n). near: cj is within one period
+   of the range (the stencil reach is below n), one conditional step. */
+__device__ __forceinline__ int wrap_cell(int cj, int n, bool near) {
+    if (near) {
+        cj += cj < 0 ? n : 0;
+        return cj >= n ? cj - n : cj;
+    }
+    return cj - floor_div(cj, n) * n;
+}
+
 /* Largest s with base[s] <= idx (base holds nsys+1 ascending offsets). */
 __device__ __forceinline__ int system_of(const int32_t *__restrict__ base, int nsys,
                                          int idx) {
@@ -492,6 +508,8 @@ struct HomeCell {
     int c0, c1, c2;        /* cell coordinates */
     int n0, n1, n2;
     int nx, ny, nz, sx, sy, nseg, centre;
+    uint32_t msx, msy;     /* division by sx, sy (make_div_magic) */
+    int hsx, hsy;
     int p0, p1, p2;
     int abase, cbase;
     double cutoff_sq;
@@ -513,9 +531,10 @@ __device__ __forceinline__ bool load_home(const PairArgs &A, int h, HomeCell &H)
     H.n2 = gp->n[2];
     if (!HASHED) {
         const int lin = h - gp->cell_base;
-        H.c0 = lin % H.n0;
-        H.c1 = (lin / H.n0) % H.n1;
-        H.c2 = lin / (H.n0 * H.n1);
+        const int q = div_magic(lin, gp->dmul[0], gp->dshift[0]);
+        H.c0 = lin - q * H.n0;
+        H.c2 = div_magic(q, gp->dmul[1], gp->dshift[1]);
+        H.c1 = q - H.c2 * H.n1;
     } else {
         const int64_t key = as_const(A.hkey)[h];
         H.c0 = static_cast<int>(key % H.n0);
@@ -527,6 +546,10 @@ __device__ __forceinline__ bool load_home(const PairArgs &A, int h, HomeCell &H)
     H.nz = gp->nxyz[2];
     H.sx = 2 * H.nx + 1;
     H.sy = 2 * H.ny + 1;
+    H.msx = gp->dmul[2];
+    H.hsx = gp->dshift[2];
+    H.msy = gp->dmul[3];
+    H.hsy = gp->dshift[3];
     H.nseg = gp->nseg;
     H.centre = (H.nz * H.sy + H.ny) * H.sx + H.nx; /* segment of offset (0,0,0) */
     H.p0 = gp->pbc[0];
@@ -568,22 +591,23 @@ __device__ __forceinline__ int build_segments(const PairArgs &A, const HomeCell 
     int len = 0, begin = 0, pxyz = 0;
     if (lane < nsg) {
         const int seg = sg + lane;
-        const int x = seg % H.sx - H.nx;
-        const int rest = seg / H.sx;
-        const int y = rest % H.sy - H.ny;
-        const int z = rest / H.sy - H.nz;
+        const int rest = div_magic(seg, H.msx, H.hsx);
+        const int x = seg - rest * H.sx - H.nx;
+        const int zq = div_magic(rest, H.msy, H.hsy);
+        const int y = rest - zq * H.sy - H.ny;
+        const int z = zq - H.nz;
         int cj0 = H.c0 + x, cj1 = H.c1 + y, cj2 = H.c2 + z;
         bool ok = true;
         if (H.p0)
-            cj0 -= floor_div(cj0, H.n0) * H.n0;
+            cj0 = wrap_cell(cj0, H.n0, H.nx < H.n0);
         else
             ok = ok && cj0 >= 0 && cj0 < H.n0;
         if (H.p1)
-            cj1 -= floor_div(cj1, H.n1) * H.n1;
+            cj1 = wrap_cell(cj1, H.n1, H.ny < H.n1);
         else
             ok = ok && cj1 >= 0 && cj1 < H.n1;
         if (H.p2)
-            cj2 -= floor_div(cj2, H.n2) * H.n2;
+            cj2 = wrap_cell(cj2, H.n2, H.nz < H.n2);
         else
             ok = ok && cj2 >= 0 && cj2 < H.n2;
         if (ok) {
@@ -1001,19 +1025,21 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L,
                 for (int t = 0; t < mg; t++) {
                     const int si = H.hb + grp0 + t;
                     const double4 pi = load_const(A.spos, si);
+                    /* atom t's pairs so far: a scalar across the window */
+                    int rank0 = __builtin_amdgcn_readlane(acc, t);
 #pragma unroll
                     for (int k = 0; k < kFillChunks; k++) {
                         const int cb = w0 + k * kWave;
                         if (cb >= total) break;
                         double d0, d1, d2;
-                        const bool ok =
-                            pair_test<CUT>(A, H, C[k], si, pi, d0, d1, d2, pt_sq) &
-                            (cb + lane != self_base + t);
-                        const uint64_t mask = __builtin_amdgcn_ballot_w64(ok);
+                        uint64_t mask = __builtin_amdgcn_ballot_w64(
+                            pair_test<CUT>(A, H, C[k], si, pi, d0, d1, d2, pt_sq));
+                        /* the atom itself at zero offset (neighbour_visit.hh:86) */
+                        const int self_lane = self_base + t - cb;
+                        if (self_lane >= 0 && self_lane < kWave) mask &= ~(1ull << self_lane);
                         if (mask) {
-                            const int r = mbcnt64(mask);
-                            const int rank = __builtin_amdgcn_readlane(acc, t) + r;
-                            if (ok) {
+                            if (__builtin_amdgcn_inverse_ballot_w64(mask)) {
+                                const int r = mbcnt64(mask);
                                 const int e = (head + r) & (kRing - 1);
                                 L.rdx[e] = d0;
                                 L.rdy[e] = d1;
@@ -1022,13 +1048,10 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L,
                                 L.rc0[e] = C[k].c0;
                                 L.rc1[e] = C[k].c1;
                                 L.rc2[e] = C[k].c2;
-                                L.rtk[e] = t | (rank << 6);
+                                L.rtk[e] = t | ((rank0 + r) << 6);
                             }
                             const int pc = static_cast<int>(__popcll(mask));
-                            acc = static_cast<int>(write_lane(
-                                static_cast<uint32_t>(acc),
-                                static_cast<uint32_t>(__builtin_amdgcn_readlane(acc, t) + pc),
-                                t));
+                            rank0 += pc;
                             head += pc;
                             if (head - tail >= kWave) {
                                 flush_ring<MODE>(A, H, L, tail, kWave);
@@ -1036,6 +1059,8 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L,
                             }
                         }
                     }
+                    acc = static_cast<int>(
+                        write_lane(static_cast<uint32_t>(acc), static_cast<uint32_t>(rank0), t));
                 }
             }
             __builtin_amdgcn_wave_barrier();
