@@ -287,8 +287,15 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
     const int4 rc = raw4[a];
     const int b = cell_first[rc.w];
     const int m = cell_count[rc.w];
+    /* members read eight at a time: independent loads, one round trip */
     int rank = 0;
-    for (int t = 0; t < m; t++) rank += tmp[b + t] < a ? 1 : 0;
+    for (int t0 = 0; t0 < m; t0 += 8) {
+        int v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = t0 + u < m ? tmp[b + t0 + u] : INT32_MAX;
+#pragma unroll
+        for (int u = 0; u < 8; u++) rank += v[u] < a ? 1 : 0;
+    }
     const int s = b + rank;
 
     const int sys = system_of(atom_base, nsys, a);
@@ -369,13 +376,33 @@ __global__ __launch_bounds__(1024) void k_scan_spine(int64_t *__restrict__ parti
     if (threadIdx.x == 0 && total2) *total2 = s_carry;
 }
 
+/* Scan of the tiles with their offsets: with `fused`, partial holds the
+   tiles' sums and each block adds up those before it (few tiles: replaces
+   the spine kernel; the last block writes the totals); otherwise partial
+   holds the exclusive offsets the spine wrote. */
 template <typename TI, typename TO>
 __global__ __launch_bounds__(kBlock) void k_scan_apply(const TI *__restrict__ in, int64_t n,
                                                         const int64_t *__restrict__ partial,
-                                                        TO *__restrict__ out, int write_total) {
+                                                        TO *__restrict__ out, int write_total,
+                                                        int fused, int64_t *__restrict__ total,
+                                                        int64_t *__restrict__ total2) {
     __shared__ int64_t s_tile[kScanTile];
     __shared__ int64_t s_w[kWavesPerBlock];
+    __shared__ int64_t s_pre;
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+    if (fused) {
+        int64_t v = 0;
+        for (int b = threadIdx.x; b < static_cast<int>(blockIdx.x); b += kBlock) v += partial[b];
+        v = wave_sum(v);
+        if (lane_id() == 0) s_w[threadIdx.x / kWave] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t t = 0;
+            for (int w = 0; w < kWavesPerBlock; w++) t += s_w[w];
+            s_pre = t;
+        }
+        __syncthreads();
+    }
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
         const int64_t idx = base + k * kBlock + threadIdx.x;
@@ -392,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const TI *__restrict__ in
     const int64_t incl = wave_incl_scan(sum);
     if (lane_id() == kWave - 1) s_w[threadIdx.x / kWave] = incl;
     __syncthreads();
-    int64_t pre = partial[blockIdx.x];
+    int64_t pre = fused ? s_pre : partial[blockIdx.x];
     for (int w = 0; w < static_cast<int>(threadIdx.x / kWave); w++) pre += s_w[w];
     pre += incl - sum;
 #pragma unroll
@@ -408,6 +435,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const TI *__restrict__ in
         const int64_t last = n - 1;
         const int64_t tot = s_tile[last - base] + static_cast<int64_t>(in[last]);
         out[n] = static_cast<TO>(tot);
+    }
+    if (fused && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1 && (total || total2)) {
+        const int64_t last = n - 1;
+        const int64_t tot = s_tile[last - base] + static_cast<int64_t>(in[last]);
+        if (total) *total = tot;
+        if (total2) *total2 = tot;
     }
 }
 
@@ -1317,9 +1350,14 @@ static hipError_t scan_impl(const TI *in, int64_t n, TO *out, int64_t *partial,
     }
     const int64_t nb = (n + kScanTile - 1) / kScanTile;
     k_scan_reduce<TI><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(in, n, partial);
-    k_scan_spine<<<1, 1024, 0, stream>>>(partial, nb, total, total2);
+    /* up to this many tiles, each apply block adds up the tile sums before
+       it (no spine kernel) */
+    constexpr int64_t kFusedTiles = 1024;
+    const bool fused = nb <= kFusedTiles;
+    if (!fused) k_scan_spine<<<1, 1024, 0, stream>>>(partial, nb, total, total2);
     k_scan_apply<TI, TO><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(
-        in, n, partial, out, write_total ? 1 : 0);
+        in, n, partial, out, write_total ? 1 : 0, fused ? 1 : 0, fused ? total : nullptr,
+        fused ? total2 : nullptr);
     return hipGetLastError();
 }
 

@@ -78,3 +78,37 @@ def test_slab_per_type_bitwise(gpu, backend):
     ref = _oracle(w, cell, inv)
     assert_same_pairs(res, ref, msg=f"slab backend {backend}")
     _properties(w, res, cell)
+
+
+def test_many_atoms_row_offsets(gpu):
+    """4.6 M atoms (more than the 4096 x 1024 counts one scan pass adds up
+    in its apply blocks, so the three-kernel scan runs), uniform at 0.01
+    atoms/A^3, rc 3 A: the rows follow the per-atom counts (i sorted, each
+    atom's row length equal to its count, the total equal to the sum), and
+    D, d are consistent with the positions."""
+    import torch
+
+    from matscipy_neighbours_amd import neighbours as nb
+    n = 4_600_000
+    L = (n / 0.01) ** (1.0 / 3.0)
+    rng = np.random.default_rng(21)
+    r = rng.uniform(0.0, L, (n, 3))
+    cell = L * np.eye(3)
+    inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+    print(f"{n} atoms, box {L:.1f} A", flush=True)
+    b = nb._Build(torch.from_numpy(r).to("cuda:0"), np.zeros(3), cell, inv,
+                  np.ones(3, bool), 3.0, None)
+    out = b.run(nb._quantity_flags("ijdDS"), want_counts=True)
+    torch.cuda.synchronize()
+    i, j, c = out["i"].cpu().numpy(), out["j"].cpu().numpy(), out["c"].cpu().numpy()
+    assert len(i) == int(c.astype(np.int64).sum()) > 0
+    assert np.all(np.diff(i) >= 0)
+    assert np.array_equal(np.bincount(i, minlength=n), c)
+    sel = rng.integers(0, len(i), 200_000)
+    D = out["D"].cpu().numpy()[sel]
+    S = out["S"].cpu().numpy()[sel]
+    ref_D = r[j[sel]] - r[i[sel]] + S @ cell
+    assert np.max(np.abs(D - ref_D)) < 1e-9
+    d = out["d"].cpu().numpy()[sel]
+    assert np.max(np.abs(d - np.linalg.norm(D, axis=1))) < 1e-12
+    assert np.all(d < 3.0)
