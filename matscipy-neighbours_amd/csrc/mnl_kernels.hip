@@ -126,14 +126,14 @@ __device__ __forceinline__ int mbcnt64(uint64_t mask) {
 }
 
 /* XCD-aware remap: the hardware deals workgroups round-robin to the 8 XCDs.
-   Hand each XCD runs of kXcdRun consecutive logical blocks (64 cells), runs
+   Hand each XCD runs of kXcdRun consecutive logical blocks (128 cells), runs
    dealt round-robin: neighbouring cells share an L2, and work that is
    concentrated in one part of the grid (a slab under vacuum) still spreads
    over all XCDs. A bijection for any grid size (the tail that does not fill
    a round of runs keeps the hardware order). */
 __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
     constexpr int kXcd = 8;
-    constexpr int kXcdRun = 16;
+    constexpr int kXcdRun = 32;
     const int full = nblocks / (kXcd * kXcdRun) * (kXcd * kXcdRun);
     if (b >= full) return b;
     const int x = b % kXcd, k = b / kXcd;
@@ -1120,10 +1120,10 @@ __global__ __launch_bounds__(kBlock) void k_fill(PairArgs A) {
     fill_cell<MODE, CUT, HASHED>(A, h, s_lds[wib], pt_sq);
 }
 
-/* Device-side parameter set-up (one thread): keeps the whole build free of
-   host-to-device copies, so that it can be captured in a graph. */
-/* One-system parameters, and the cleared grid: cell (or slot) counts to 0,
-   hash keys (if any) to empty. */
+/* Device-side parameter set-up (keeps the whole build free of host-to-device
+   copies, so that it can be captured in a graph): the one-system parameters,
+   and the cleared grid: cell (or slot) counts to 0, hash keys (if any) to
+   empty. */
 __global__ void k_set_params(SysGeom *__restrict__ geo, SysGeom g, int32_t *__restrict__ atom_base,
                              int32_t *__restrict__ cell_base, int32_t nat, int32_t ncells,
                              int32_t *__restrict__ cell_count, int64_t nslots,
