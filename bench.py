@@ -393,9 +393,9 @@ def main():
     if args.traffic_from:
         traffic_summary(*args.traffic_from, commands=[
             "rocprofv3 --pmc FETCH_SIZE -f csv -d <dir> -o run -- python3 bench.py --steps 3 "
-            "--warmup 1 --no-cpu",
+            "--warmup 1 --no-cpu --no-batch --no-extra",
             "rocprofv3 --pmc WRITE_SIZE -f csv -d <dir> -o run -- python3 bench.py --steps 3 "
-            "--warmup 1 --no-cpu",
+            "--warmup 1 --no-cpu --no-batch --no-extra",
             "rocprofv3 --pmc FETCH_SIZE -f csv -d <dir> -o run -- python3 bench.py "
             "--calibrate-copy --steps 3",
             "rocprofv3 --pmc WRITE_SIZE -f csv -d <dir> -o run -- python3 bench.py "
