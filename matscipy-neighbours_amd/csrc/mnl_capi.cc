@@ -1075,7 +1075,7 @@ int mnl_plan_pair_backward(mnl_plan *p, void *stream, int64_t npairs, int32_t na
     DeviceScope ds(p->device);
     if (!ds.ok) return fail("cannot switch to the plan's device");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const size_t need = sizeof(double) * 9 * static_cast<size_t>(backward_blocks(npairs));
+    const size_t need = sizeof(double) * static_cast<size_t>(backward_scratch_doubles(nat));
     if (need > p->cell_partial.cap) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         MNL_HIP(hipStreamIsCapturing(st, &cs));

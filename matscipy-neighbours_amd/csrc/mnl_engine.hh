@@ -159,10 +159,13 @@ hipError_t launch_set_params(SysGeom *geo, const SysGeom &g, int32_t *atom_base,
 hipError_t launch_sqrt_probe(const double *x, double *y, int64_t n,
                              hipStream_t stream);
 /* Backward of D = r[j] - r[i] + S.cell, d = |D| (see mnl_kernels.hip):
-   overwrites grad_pos [nat*3] and grad_cell [9]; cell_partial needs
-   backward_blocks(npairs) * 9 doubles. D, d, gD, gd are float when f32. */
+   overwrites grad_pos [nat*3] and grad_cell [9]; cell_partial is scratch of
+   backward_scratch_doubles(nat) doubles. D, d, gD, gd are float when f32. */
 constexpr int kBackwardMaxBlocks = 2048;
 int backward_blocks(int64_t npairs);
+inline int64_t backward_scratch_doubles(int32_t nat) {
+    return 9 * int64_t(kBackwardMaxBlocks) + 3 * int64_t(nat);
+}
 hipError_t launch_pair_backward(int64_t npairs, int32_t nat, const int32_t *first,
                                 const int32_t *secnd, const int32_t *shift, const void *D,
                                 const void *d, const void *gD, const void *gd, bool f32,

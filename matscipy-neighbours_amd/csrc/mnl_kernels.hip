@@ -1169,8 +1169,8 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pair_backward(
     int64_t npairs, const int32_t *__restrict__ first, const int32_t *__restrict__ secnd,
     const int32_t *__restrict__ shift, const T *__restrict__ D, const T *__restrict__ d,
-    const T *__restrict__ gD, const T *__restrict__ gd, double *__restrict__ grad_pos,
-    double *__restrict__ cell_partial) {
+    const T *__restrict__ gD, const T *__restrict__ gd, double *__restrict__ gx,
+    double *__restrict__ gy, double *__restrict__ gz, double *__restrict__ cell_partial) {
     __shared__ double s_red[kWavesPerBlock][9];
     double c[9];
 #pragma unroll
@@ -1205,9 +1205,9 @@ __global__ __launch_bounds__(kBlock) void k_pair_backward(
             s0 = static_cast<double>(shift[3 * p + 0]);
             s1 = static_cast<double>(shift[3 * p + 1]);
             s2 = static_cast<double>(shift[3 * p + 2]);
-            unsafeAtomicAdd(&grad_pos[3 * static_cast<int64_t>(j) + 0], g0);
-            unsafeAtomicAdd(&grad_pos[3 * static_cast<int64_t>(j) + 1], g1);
-            unsafeAtomicAdd(&grad_pos[3 * static_cast<int64_t>(j) + 2], g2);
+            unsafeAtomicAdd(&gx[j], g0);
+            unsafeAtomicAdd(&gy[j], g1);
+            unsafeAtomicAdd(&gz[j], g2);
         }
         c[0] += s0 * g0; c[1] += s0 * g1; c[2] += s0 * g2;
         c[3] += s1 * g0; c[4] += s1 * g1; c[5] += s1 * g2;
@@ -1218,9 +1218,9 @@ __global__ __launch_bounds__(kBlock) void k_pair_backward(
         const int next = __shfl_down(i, 1, kWave);
         const bool tail = lane_id() == kWave - 1 || next != i;
         if (valid && tail) {
-            unsafeAtomicAdd(&grad_pos[3 * static_cast<int64_t>(i) + 0], -a0);
-            unsafeAtomicAdd(&grad_pos[3 * static_cast<int64_t>(i) + 1], -a1);
-            unsafeAtomicAdd(&grad_pos[3 * static_cast<int64_t>(i) + 2], -a2);
+            unsafeAtomicAdd(&gx[i], -a0);
+            unsafeAtomicAdd(&gy[i], -a1);
+            unsafeAtomicAdd(&gz[i], -a2);
         }
     }
 #pragma unroll
@@ -1236,6 +1236,17 @@ __global__ __launch_bounds__(kBlock) void k_pair_backward(
         for (int q = 0; q < kWavesPerBlock; q++) t += s_red[q][threadIdx.x];
         cell_partial[static_cast<int64_t>(blockIdx.x) * 9 + threadIdx.x] = t;
     }
+}
+
+/* grad_pos[a] = (gx[a], gy[a], gz[a]): the per-component sums, accumulated
+   in separate arrays so that a wave's atomics on consecutive atoms land in
+   few cache lines, interleaved into the caller's [nat][3] layout. */
+__global__ __launch_bounds__(kBlock) void k_soa_to_aos(const double *__restrict__ soa,
+                                                       int32_t nat, double *__restrict__ out) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= 3 * static_cast<int64_t>(nat)) return;
+    const int64_t a = t / 3, k = t - 3 * a;
+    out[t] = soa[k * nat + a];
 }
 
 /* Sum of nb per-block cell partials in a fixed order (one block): each
@@ -1464,25 +1475,29 @@ hipError_t launch_pair_backward(int64_t npairs, int32_t nat, const int32_t *firs
                                 const void *d, const void *gD, const void *gd, bool f32,
                                 double *grad_pos, double *grad_cell, double *cell_partial,
                                 hipStream_t stream) {
-    if (nat > 0)
-        k_fill_f64<<<grid_for(int64_t(nat) * 3), kBlock, 0, stream>>>(grad_pos,
-                                                                     int64_t(nat) * 3, 0.0);
     const int nb = backward_blocks(npairs);
+    /* per-component sums after the cell partials */
+    double *soa = cell_partial + 9 * static_cast<int64_t>(kBackwardMaxBlocks);
+    const int64_t n3 = int64_t(nat) * 3;
+    if (nat > 0) k_fill_f64<<<grid_for(n3), kBlock, 0, stream>>>(soa, n3, 0.0);
     if (npairs > 0) {
         if (f32)
             k_pair_backward<float><<<nb, kBlock, 0, stream>>>(
                 npairs, first, secnd, shift, static_cast<const float *>(D),
                 static_cast<const float *>(d), static_cast<const float *>(gD),
-                static_cast<const float *>(gd), grad_pos, cell_partial);
+                static_cast<const float *>(gd), soa, soa + nat, soa + 2 * int64_t(nat),
+                cell_partial);
         else
             k_pair_backward<double><<<nb, kBlock, 0, stream>>>(
                 npairs, first, secnd, shift, static_cast<const double *>(D),
                 static_cast<const double *>(d), static_cast<const double *>(gD),
-                static_cast<const double *>(gd), grad_pos, cell_partial);
+                static_cast<const double *>(gd), soa, soa + nat, soa + 2 * int64_t(nat),
+                cell_partial);
         k_cell_reduce<<<1, kBlock, 0, stream>>>(cell_partial, nb, grad_cell);
     } else {
         k_fill_f64<<<1, kWave, 0, stream>>>(grad_cell, 9, 0.0);
     }
+    if (nat > 0) k_soa_to_aos<<<grid_for(n3), kBlock, 0, stream>>>(soa, nat, grad_pos);
     return hipGetLastError();
 }
 
