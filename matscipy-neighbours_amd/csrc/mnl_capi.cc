@@ -121,6 +121,7 @@ struct mnl_plan {
     int device = 0;
     /* workspaces */
     DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, scount, counts,
+        order_stats,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
         htypes, cell_partial, tri_m, tri_cnt, tri_off, tri_vidx, tri_flag;
     int64_t *host_total = nullptr; /* pinned */
@@ -210,6 +211,7 @@ int ensure_workspaces(mnl_plan *p, hipStream_t stream, int32_t nat, int64_t cell
         {&p->cell_base, (static_cast<size_t>(nsys) + 1) * sizeof(int32_t)},
         {&p->total, sizeof(int64_t)},
         {&p->hkey, hashed ? static_cast<size_t>(cells) * sizeof(int64_t) : 0},
+        {&p->order_stats, 3 * sizeof(int32_t)},
     };
     bool grow = false;
     for (const Need &nd : needs) grow = grow || nd.bytes > nd.b->cap;
@@ -222,6 +224,8 @@ int ensure_workspaces(mnl_plan *p, hipStream_t stream, int32_t nat, int64_t cell
     MNL_HIP(hipStreamSynchronize(stream));
     for (const Need &nd : needs) MNL_HIP(nd.b->ensure(nd.bytes));
     p->pt_host.clear(); /* buffers may have moved: upload the table again */
+    /* no atom order seen yet: the first build keeps the default layout */
+    MNL_HIP(launch_fill_i32(p->order_stats.as<int32_t>(), 3, 0, stream));
     return MNL_SUCCESS;
 }
 
@@ -242,9 +246,12 @@ int enqueue_binning(mnl_plan *p, hipStream_t stream, const double *positions,
                                   p->ticket.as<int32_t>(), p->hkey.as<int64_t>(), p->hmask,
                                   p->cell_count.as<int32_t>(), stream));
     } else {
+        /* one system: count the input's order for the next build's layout
+           (batched geometry keeps the default layout) */
         MNL_HIP(launch_bin_dense(positions, nat, p->geo.as<SysGeom>(),
                                  p->atom_base.as<int32_t>(), p->nsys, p->raw4.as<int4>(),
                                  p->ticket.as<int32_t>(), p->cell_count.as<int32_t>(),
+                                 p->batch_ready ? nullptr : p->order_stats.as<int32_t>(),
                                  stream));
     }
     MNL_HIP(launch_scan_i32_to_i32(p->cell_count.as<int32_t>(), cells,
@@ -344,7 +351,8 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
                               p->cell_base.as<int32_t>(), nat,
                               static_cast<int32_t>(hashed ? 0 : ncells),
                               p->cell_count.as<int32_t>(), cells_or_slots,
-                              hashed ? p->hkey.as<int64_t>() : nullptr, stream));
+                              hashed ? p->hkey.as<int64_t>() : nullptr,
+                              hashed ? nullptr : p->order_stats.as<int32_t>(), stream));
     if (p->cut == CUT_PER_TYPE && ncutoffs > 0) {
         /* the table is uploaded only when it differs from the plan's copy, so
            steady-state builds with fixed cutoffs stay asynchronous */
@@ -615,7 +623,7 @@ void mnl_plan_destroy(mnl_plan *p) {
     if (!p) return;
     DeviceScope ds(p->device);
     DevBuf *bufs[] = {&p->raw4,      &p->ticket,    &p->tmp,    &p->spos,     &p->sinfo,
-                      &p->stype,     &p->cell_count, &p->cell_first, &p->scount, &p->counts, &p->offsets,
+                      &p->stype,     &p->cell_count, &p->cell_first, &p->scount, &p->counts, &p->order_stats, &p->offsets,
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
                       &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
                       &p->cell_partial, &p->tri_m,   &p->tri_cnt,  &p->tri_off,

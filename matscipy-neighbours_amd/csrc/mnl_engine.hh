@@ -41,11 +41,15 @@ struct SysGeom {
     int32_t cell_base;  /* first dense cell of the system */
     int32_t ncells;
     int32_t nseg;       /* (2nx+1)(2ny+1)(2nz+1) neighbour-cell offsets */
-    /* division by n[0], n[1], 2nx+1, 2ny+1 as a multiply and shift:
+    /* dense cell layout: cell = c[f] + n[f] * (c[m] + n[m] * c[s]) with
+       (f, m, s) = lay; {0, 1, 2} (x fastest) unless the binning of the
+       previous build found the input's atom order running along another
+       axis (k_set_params). Internal: no output depends on it. */
+    int32_t lay[3];
+    /* division by n[0], n[1], n[2], 2nx+1, 2ny+1 as a multiply and shift:
        x / d == (x * dmul) >> dshift for 0 <= x < 2^31 (make_div_magic) */
-    uint32_t dmul[4];
-    int32_t dshift[4];
-    int64_t pad;
+    uint32_t dmul[5];
+    int32_t dshift[5];
 };
 
 /* Multiplier and shift with x / d == (uint64(x) * mul) >> shift for all
@@ -101,9 +105,11 @@ struct PairArgs {
 /* Kernel launchers (mnl_kernels.hip). All asynchronous on `stream`. */
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t stream);
 hipError_t launch_fill_i64(int64_t *p, int64_t n, int64_t v, hipStream_t stream);
+/* order_stats (may be NULL): per axis, the number of consecutive atoms
+   whose cells differ along it, added to (for the next build's layout). */
 hipError_t launch_bin_dense(const double *r, int32_t nat, const SysGeom *geo,
                             const int32_t *atom_base, int32_t nsys, int4 *raw4,
-                            int32_t *ticket, int32_t *cell_count,
+                            int32_t *ticket, int32_t *cell_count, int32_t *order_stats,
                             hipStream_t stream);
 hipError_t launch_bin_hashed(const double *r, int32_t nat, const SysGeom *geo,
                              int4 *raw4, int32_t *ticket, int64_t *hkey,
@@ -151,11 +157,13 @@ hipError_t launch_triplet_fill(const int32_t *first_i, int32_t nrows, const int3
 hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
                         hipStream_t stream);
 /* One-system parameters into geo/atom_base/cell_base, and the grid cleared:
-   cell_count[0..nslots) = 0 and, if hkey is given, hkey[0..nslots) = -1. */
+   cell_count[0..nslots) = 0 and, if hkey is given, hkey[0..nslots) = -1.
+   order_stats (may be NULL): the previous binning's per-axis counts, which
+   choose the dense layout, then cleared. */
 hipError_t launch_set_params(SysGeom *geo, const SysGeom &g, int32_t *atom_base,
                              int32_t *cell_base, int32_t nat, int32_t ncells,
                              int32_t *cell_count, int64_t nslots, int64_t *hkey,
-                             hipStream_t stream);
+                             int32_t *order_stats, hipStream_t stream);
 hipError_t launch_sqrt_probe(const double *x, double *y, int64_t n,
                              hipStream_t stream);
 /* Backward of D = r[j] - r[i] + S.cell, d = |D| (see mnl_kernels.hip):

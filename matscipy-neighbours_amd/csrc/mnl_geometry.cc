@@ -74,9 +74,10 @@ bool make_sys_geom(const double origin[3], const double cell[9],
     const int64_t nc = static_cast<int64_t>(g.n[0]) * g.n[1] * g.n[2];
     g.ncells = nc > INT32_MAX ? INT32_MAX : static_cast<int32_t>(nc);
     g.nseg = (2 * g.nxyz[0] + 1) * (2 * g.nxyz[1] + 1) * (2 * g.nxyz[2] + 1);
-    const int32_t divisors[4] = {g.n[0], g.n[1], 2 * g.nxyz[0] + 1, 2 * g.nxyz[1] + 1};
-    for (int k = 0; k < 4; k++) make_div_magic(divisors[k], g.dmul[k], g.dshift[k]);
-    g.pad = 0;
+    for (int k = 0; k < 3; k++) g.lay[k] = k;
+    const int32_t divisors[5] = {g.n[0], g.n[1], g.n[2], 2 * g.nxyz[0] + 1,
+                                 2 * g.nxyz[1] + 1};
+    for (int k = 0; k < 5; k++) make_div_magic(divisors[k], g.dmul[k], g.dshift[k]);
     return true;
 }
 

@@ -78,6 +78,20 @@ __device__ __forceinline__ int wrap_cell(int cj, int n, bool near) {
     return cj - floor_div(cj, n) * n;
 }
 
+/* Component k of (a, b, c). */
+__device__ __forceinline__ int pick3(int k, int a, int b, int c) {
+    return k == 0 ? a : (k == 1 ? b : c);
+}
+
+/* Dense cell index (without the system's base) of wrapped cell coordinates
+   in the system's layout lay = (f, m, s): c[f] + n[f] * (c[m] + n[m] * c[s]). */
+__device__ __forceinline__ int dense_index(const SysGeom &g, int c0, int c1, int c2) {
+    const int f = g.lay[0], m = g.lay[1], sl = g.lay[2];
+    return pick3(f, c0, c1, c2) +
+           pick3(f, g.n[0], g.n[1], g.n[2]) *
+               (pick3(m, c0, c1, c2) + pick3(m, g.n[0], g.n[1], g.n[2]) * pick3(sl, c0, c1, c2));
+}
+
 /* Largest s with base[s] <= idx (base holds nsys+1 ascending offsets). */
 __device__ __forceinline__ int system_of(const int32_t *__restrict__ base, int nsys,
                                          int idx) {
@@ -197,19 +211,38 @@ __device__ __forceinline__ int wave_ticket(int32_t *__restrict__ counters, int k
 __global__ __launch_bounds__(kBlock) void k_bin_dense(
     const double *__restrict__ r, int32_t nat, const SysGeom *__restrict__ geo,
     const int32_t *__restrict__ atom_base, int32_t nsys, int4 *__restrict__ raw4,
-    int32_t *__restrict__ ticket, int32_t *__restrict__ cell_count) {
+    int32_t *__restrict__ ticket, int32_t *__restrict__ cell_count,
+    int32_t *__restrict__ order_stats) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = a < nat;
     int raw[3] = {0, 0, 0};
+    int c[3] = {0, 0, 0};
     int cell = -1;
     if (valid) {
         const SysGeom &g = geo[system_of(atom_base, nsys, a)];
         raw_cell_of(r, a, g, raw);
-        int c[3];
 #pragma unroll
         for (int k = 0; k < 3; k++)
             c[k] = g.pbc[k] ? wrap_idx(raw[k], g.n[k]) : clamp_idx(raw[k], g.n[k]);
-        cell = g.cell_base + c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
+        cell = g.cell_base + dense_index(g, c[0], c[1], c[2]);
+    }
+    if (order_stats && blockIdx.x % 16 == 0 && threadIdx.x < kWave) {
+        /* A sample (the first wave of every 16th block; all waves adding to
+           the same three counters would serialise): per axis, the periodic
+           cell distance between atoms a and a + 32. Over 32 atoms a lattice
+           in index order moves along its fastest axis only (32 is a multiple
+           of the usual basis sizes), a random order along all three. */
+        const SysGeom &g0 = geo[0];
+        const bool pair = lane_id() < 32 && a + 32 < nat;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int cn = __shfl_down(c[k], 32, kWave);
+            int dd = cn - c[k];
+            if (2 * dd > g0.n[k]) dd -= g0.n[k];
+            if (2 * dd < -g0.n[k]) dd += g0.n[k];
+            const int moved = wave_sum(pair ? (dd < 0 ? -dd : dd) : 0);
+            if (lane_id() == 0 && moved) atomicAdd(&order_stats[k], moved);
+        }
     }
     const int tk = wave_ticket(cell_count, cell, valid);
     if (!valid) return;
@@ -459,9 +492,11 @@ __device__ __forceinline__ void unpack3(int p, int &a, int &b, int &c) {
 }
 
 /* Dense slice of cell (c0,c1,c2) of a system (cell_list.hh:94-104). */
-__device__ __forceinline__ void dense_slice(const PairArgs &A, int cell_base, int n0, int n1,
-                                            int c0, int c1, int c2, int &b, int &len) {
-    const int c = cell_base + c0 + n0 * (c1 + n1 * c2);
+__device__ __forceinline__ void dense_slice(const PairArgs &A, int cell_base, int lf, int lm,
+                                            int ls, int nf, int nm, int c0, int c1, int c2,
+                                            int &b, int &len) {
+    const int c = cell_base + pick3(lf, c0, c1, c2) +
+                  nf * (pick3(lm, c0, c1, c2) + nm * pick3(ls, c0, c1, c2));
     b = A.cell_first[c];
     len = A.cell_count[c];
 }
@@ -545,6 +580,7 @@ struct HomeCell {
     int hsx, hsy;
     int p0, p1, p2;
     int abase, cbase;
+    int lf, lm, ls, nf, nm; /* dense layout: fastest, middle, slowest axis */
     double cutoff_sq;
     double b00, b01, b02, b10, b11, b12, b20, b21, b22;
     double rn0, rn1, rn2;
@@ -562,12 +598,21 @@ __device__ __forceinline__ bool load_home(const PairArgs &A, int h, HomeCell &H)
     H.n0 = gp->n[0];
     H.n1 = gp->n[1];
     H.n2 = gp->n[2];
+    H.lf = gp->lay[0];
+    H.lm = gp->lay[1];
+    H.ls = gp->lay[2];
+    H.nf = pick3(H.lf, H.n0, H.n1, H.n2);
+    H.nm = pick3(H.lm, H.n0, H.n1, H.n2);
     if (!HASHED) {
+        /* lin = c_f + n_f (c_m + n_m c_s) */
         const int lin = h - gp->cell_base;
-        const int q = div_magic(lin, gp->dmul[0], gp->dshift[0]);
-        H.c0 = lin - q * H.n0;
-        H.c2 = div_magic(q, gp->dmul[1], gp->dshift[1]);
-        H.c1 = q - H.c2 * H.n1;
+        const int q = div_magic(lin, gp->dmul[H.lf], gp->dshift[H.lf]);
+        const int cf = lin - q * H.nf;
+        const int cs = div_magic(q, gp->dmul[H.lm], gp->dshift[H.lm]);
+        const int cm = q - cs * H.nm;
+        H.c0 = H.lf == 0 ? cf : (H.lm == 0 ? cm : cs);
+        H.c1 = H.lf == 1 ? cf : (H.lm == 1 ? cm : cs);
+        H.c2 = H.lf == 2 ? cf : (H.lm == 2 ? cm : cs);
     } else {
         const int64_t key = as_const(A.hkey)[h];
         H.c0 = static_cast<int>(key % H.n0);
@@ -579,10 +624,10 @@ __device__ __forceinline__ bool load_home(const PairArgs &A, int h, HomeCell &H)
     H.nz = gp->nxyz[2];
     H.sx = 2 * H.nx + 1;
     H.sy = 2 * H.ny + 1;
-    H.msx = gp->dmul[2];
-    H.hsx = gp->dshift[2];
-    H.msy = gp->dmul[3];
-    H.hsy = gp->dshift[3];
+    H.msx = gp->dmul[3];
+    H.hsx = gp->dshift[3];
+    H.msy = gp->dmul[4];
+    H.hsy = gp->dshift[4];
     H.nseg = gp->nseg;
     H.centre = (H.nz * H.sy + H.ny) * H.sx + H.nx; /* segment of offset (0,0,0) */
     H.p0 = gp->pbc[0];
@@ -647,7 +692,8 @@ __device__ __forceinline__ int build_segments(const PairArgs &A, const HomeCell 
             if (HASHED)
                 hashed_slice(A, H.n0, H.n1, cj0, cj1, cj2, begin, len);
             else
-                dense_slice(A, H.cbase, H.n0, H.n1, cj0, cj1, cj2, begin, len);
+                dense_slice(A, H.cbase, H.lf, H.lm, H.ls, H.nf, H.nm, cj0, cj1, cj2, begin,
+                            len);
         }
         pxyz = pack3(x, y, z);
         /* the offset vector, neighbour_visit.hh:67,73-74,80-81 */
@@ -1127,13 +1173,32 @@ __global__ __launch_bounds__(kBlock) void k_fill(PairArgs A) {
 __global__ void k_set_params(SysGeom *__restrict__ geo, SysGeom g, int32_t *__restrict__ atom_base,
                              int32_t *__restrict__ cell_base, int32_t nat, int32_t ncells,
                              int32_t *__restrict__ cell_count, int64_t nslots,
-                             int64_t *__restrict__ hkey) {
+                             int64_t *__restrict__ hkey, int32_t *__restrict__ order_stats) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t < nslots) {
         cell_count[t] = 0;
         if (hkey) hkey[t] = -1;
     }
     if (t != 0) return;
+    if (order_stats) {
+        /* The previous binning measured, per axis, how far atoms 32 indices
+           apart lie (in cells). If one axis carries at least twice the
+           distance of each other, the input's atom order runs along it:
+           make it the fastest of the dense layout (then the next), so that
+           cells processed one after another hold atoms with nearby indices
+           and the output rows they write lie close together. */
+        const int s0 = order_stats[0], s1 = order_stats[1], s2 = order_stats[2];
+        const int f = (s0 >= s1 && s0 >= s2) ? 0 : (s1 >= s2 ? 1 : 2);
+        const int u = f == 0 ? 1 : 0, v = f == 2 ? 1 : 2; /* the other two, u < v */
+        const int sf = pick3(f, s0, s1, s2), su = pick3(u, s0, s1, s2),
+                  sv = pick3(v, s0, s1, s2);
+        if (sf > 2 * su && sf > 2 * sv) {
+            g.lay[0] = f;
+            g.lay[1] = su >= sv ? u : v;
+            g.lay[2] = su >= sv ? v : u;
+        }
+        order_stats[0] = order_stats[1] = order_stats[2] = 0;
+    }
     geo[0] = g;
     atom_base[0] = 0;
     atom_base[1] = nat;
@@ -1303,10 +1368,11 @@ hipError_t launch_fill_i64(int64_t *p, int64_t n, int64_t v, hipStream_t stream)
 
 hipError_t launch_bin_dense(const double *r, int32_t nat, const SysGeom *geo,
                             const int32_t *atom_base, int32_t nsys, int4 *raw4,
-                            int32_t *ticket, int32_t *cell_count, hipStream_t stream) {
+                            int32_t *ticket, int32_t *cell_count, int32_t *order_stats,
+                            hipStream_t stream) {
     if (nat <= 0) return hipSuccess;
     k_bin_dense<<<grid_for(nat), kBlock, 0, stream>>>(r, nat, geo, atom_base, nsys, raw4,
-                                                      ticket, cell_count);
+                                                      ticket, cell_count, order_stats);
     return hipGetLastError();
 }
 
@@ -1451,9 +1517,9 @@ hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
 hipError_t launch_set_params(SysGeom *geo, const SysGeom &g, int32_t *atom_base,
                              int32_t *cell_base, int32_t nat, int32_t ncells,
                              int32_t *cell_count, int64_t nslots, int64_t *hkey,
-                             hipStream_t stream) {
+                             int32_t *order_stats, hipStream_t stream) {
     k_set_params<<<grid_for(std::max<int64_t>(nslots, 1)), kBlock, 0, stream>>>(
-        geo, g, atom_base, cell_base, nat, ncells, cell_count, nslots, hkey);
+        geo, g, atom_base, cell_base, nat, ncells, cell_count, nslots, hkey, order_stats);
     return hipGetLastError();
 }
 
