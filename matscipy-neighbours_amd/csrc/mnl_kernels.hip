@@ -240,7 +240,8 @@ __global__ __launch_bounds__(kBlock) void k_bin_dense(
             int dd = cn - c[k];
             if (2 * dd > g0.n[k]) dd -= g0.n[k];
             if (2 * dd < -g0.n[k]) dd += g0.n[k];
-            const int moved = wave_sum(pair ? (dd < 0 ? -dd : dd) : 0);
+            /* capped at 64 cells: the sums stay far below 2^31 */
+            const int moved = wave_sum(pair ? min(dd < 0 ? -dd : dd, 64) : 0);
             if (lane_id() == 0 && moved) atomicAdd(&order_stats[k], moved);
         }
     }
