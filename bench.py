@@ -28,7 +28,7 @@ METRIC = "pairs/s at 1M atoms rc=5Å (%HBM peak); batched systems/s vs per-syste
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fill_fcc1M.json")
 KERNEL_SOURCE = os.path.join(ROOT, "matscipy-neighbours_amd", "csrc", "mnl_kernels.hip")
-FILL_KERNEL = "k_fill<1, 0, false>"
+FILL_KERNEL = "k_fill<1, 0, false, 5>"
 
 
 def _source_sha():

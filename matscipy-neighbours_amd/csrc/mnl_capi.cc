@@ -135,6 +135,7 @@ struct mnl_plan {
     int64_t hmask = 0;
     int64_t npairs = 0;
     int32_t ncut = 0;
+    int32_t fill_chunks = 4; /* fill window (chunks of 64 candidates) */
     hipStream_t stream = nullptr;
     /* host copy of the per-type table on the device */
     std::vector<double> pt_host;
@@ -165,6 +166,7 @@ PairArgs pair_args(const mnl_plan *p) {
     a.stype = p->cut == CUT_PER_TYPE ? p->stype.as<int32_t>() : nullptr;
     a.pt_sq = p->cut == CUT_PER_TYPE ? p->pt.as<double>() : nullptr;
     a.ncut = p->ncut;
+    a.fill_chunks = p->fill_chunks;
     a.atom_base = p->atom_base.as<int32_t>();
     a.nat = p->nat;
     /* the count visits half the offsets when the test is symmetric in i, j */
@@ -340,6 +342,13 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
         p->hmask = 0;
     }
     p->home_cells = static_cast<int32_t>(cells_or_slots);
+    /* a 5-chunk fill window when it covers the typical home cell's
+       candidates (nseg cells at the mean density) in one pass */
+    {
+        const double cand = static_cast<double>(g.nseg) * nat /
+                            static_cast<double>(std::max<int64_t>(ncells, 1));
+        p->fill_chunks = (cand > 4 * 64 && cand <= 5 * 64) ? 5 : 4;
+    }
 
     if (ensure_workspaces(p, stream, nat, cells_or_slots, hashed, p->cut, ncutoffs, 1) !=
         MNL_SUCCESS)
@@ -940,6 +949,7 @@ int mnl_plan_set_batch(mnl_plan *p, void *stream, int32_t nsys, const int32_t *a
     MNL_HIP(hipStreamSynchronize(st));
     p->nat = nat;
     p->nsys = nsys;
+    p->fill_chunks = 4;
     p->backend = MNL_BACKEND_DENSE;
     p->cut = CUT_GLOBAL;
     p->ncut = 0;

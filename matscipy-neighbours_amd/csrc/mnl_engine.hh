@@ -86,6 +86,7 @@ struct PairArgs {
     const double *pt_sq;
     int32_t ncut;
     int32_t xcd_blocks;        /* grid size, for the XCD-aware block remap */
+    int32_t fill_chunks;       /* fill: candidate chunks per register window (4 or 5) */
     const int32_t *atom_base;  /* [nsys+1] */
     int32_t nat;               /* atoms of all systems */
     int32_t half;              /* count: visit half the offsets (symmetric test) */

@@ -968,6 +968,7 @@ __global__ __launch_bounds__(kBlock) void k_count(PairArgs A) {
 
 /* Candidate chunks held in registers while the home atoms are swept, and
    the ring of accepted pairs waiting for a full 64-lane store. */
+/* (kFillChunks is the default; a fill launch may take 5, see launch_fill_t) */
 constexpr int kFillChunks = 4;
 constexpr int kRing = 128;
 
@@ -1058,7 +1059,7 @@ __device__ __forceinline__ void flush_ring(const PairArgs &A, const HomeCell &H,
 }
 
 /*
- * Fill pass, same traversal as the count. Per window of up to kFillChunks
+ * Fill pass, same traversal as the count. Per window of up to FC
  * candidate chunks (held in registers) the wave sweeps the home atoms and,
  * for each atom, the window's chunks in order: the test is repeated and the
  * accepted lanes append (dr, j, shift part, atom, rank) to an LDS ring - atom
@@ -1067,7 +1068,7 @@ __device__ __forceinline__ void flush_ring(const PairArgs &A, const HomeCell &H,
  * per output array, at the atom's row offset plus the pair's rank. Per-atom
  * ranks continue across windows, so an atom's row fills front to back.
  */
-template <int MODE, int CUT, bool HASHED>
+template <int MODE, int CUT, bool HASHED, int FC>
 __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L,
                                           lds_dptr pt_sq) {
     HomeCell H;
@@ -1095,10 +1096,10 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L,
             int self_start;
             const int total = build_segments<HASHED>(A, H, sg, nsg, lane, L.seg, &self_start);
             const int self_base = self_start >= 0 ? self_start + grp0 : -(1 << 30);
-            for (int w0 = 0; w0 < total; w0 += kWave * kFillChunks) {
-                Cand C[kFillChunks];
+            for (int w0 = 0; w0 < total; w0 += kWave * FC) {
+                Cand C[FC];
 #pragma unroll
-                for (int k = 0; k < kFillChunks; k++)
+                for (int k = 0; k < FC; k++)
                     if (w0 + k * kWave < total)
                         load_candidate<CUT, true>(A, H, L.seg, nsg, w0 + k * kWave + lane,
                                                   total, C[k]);
@@ -1108,7 +1109,7 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L,
                     /* atom t's pairs so far: a scalar across the window */
                     int rank0 = __builtin_amdgcn_readlane(acc, t);
 #pragma unroll
-                    for (int k = 0; k < kFillChunks; k++) {
+                    for (int k = 0; k < FC; k++) {
                         const int cb = w0 + k * kWave;
                         if (cb >= total) break;
                         double d0, d1, d2;
@@ -1155,7 +1156,7 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L,
     }
 }
 
-template <int MODE, int CUT, bool HASHED>
+template <int MODE, int CUT, bool HASHED, int FC>
 __global__ __launch_bounds__(kBlock) void k_fill(PairArgs A) {
     __shared__ FillLds s_lds[kWavesPerBlock];
     __shared__ double s_pt[CUT == CUT_PER_TYPE_LDS ? kLdsTypes : 1];
@@ -1164,7 +1165,7 @@ __global__ __launch_bounds__(kBlock) void k_fill(PairArgs A) {
     const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
     const int h = blk * kWavesPerBlock + wib;
     if (h >= A.home_cells) return;
-    fill_cell<MODE, CUT, HASHED>(A, h, s_lds[wib], pt_sq);
+    fill_cell<MODE, CUT, HASHED, FC>(A, h, s_lds[wib], pt_sq);
 }
 
 /* Device-side parameter set-up (keeps the whole build free of host-to-device
@@ -1465,9 +1466,15 @@ static void launch_count_t(const PairArgs &a, unsigned nblocks, hipStream_t stre
     }
 }
 
+/* Windows of 5 chunks when the host expects 257-320 candidates per home
+   cell (one window then covers a typical cell instead of a 4-chunk window
+   and a second sweep over the home atoms for the rest); 4 otherwise. */
 template <int MODE, int CUT, bool HASHED>
 static void launch_fill_t(const PairArgs &a, unsigned nblocks, hipStream_t stream) {
-    k_fill<MODE, CUT, HASHED><<<nblocks, kBlock, 0, stream>>>(a);
+    if (a.fill_chunks == 5 && MODE != MODE_MATRIX)
+        k_fill<MODE, CUT, HASHED, 5><<<nblocks, kBlock, 0, stream>>>(a);
+    else
+        k_fill<MODE, CUT, HASHED, kFillChunks><<<nblocks, kBlock, 0, stream>>>(a);
 }
 
 template <int CUT>

@@ -276,3 +276,36 @@ def test_layout_follows_atom_order(gpu, order):
     cell = np.diag(np.asarray(reps, float) * a)
     for rep in range(4):
         _check(pos, cell, True, 4.2, msg=f"{order} build {rep}")
+
+
+@pytest.mark.parametrize("cut", ["global", "per_atom", "per_type", "f32"])
+def test_five_chunk_fill_window(gpu, cut):
+    """A density at which a home cell has 257-320 candidates (fcc 14^3,
+    a = 3.615, rc 5: 10 cells a side, ~296 candidates), so the fill takes
+    its 5-chunk window: uniform, per-atom and per-type cutoffs (dense and
+    hashed grids) bit for bit, float32 output to its tolerance."""
+    a, reps = 3.615, 14
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    g = np.stack(np.meshgrid(*[np.arange(reps)] * 3, indexing="ij"), -1).reshape(-1, 1, 3)
+    pos = ((g + basis[None]) * a).reshape(-1, 3)
+    rng = np.random.default_rng(23)
+    pos = pos + rng.normal(0.0, 0.1, pos.shape)
+    cell = reps * a * np.eye(3)
+    nat = len(pos)
+    assert 256 < 27 * nat / 10 ** 3 <= 320
+    if cut == "global":
+        _check(pos, cell, True, 5.0, backends=(0, 1))
+    elif cut == "per_atom":
+        _check(pos, cell, True, rng.uniform(2.2, 2.5, nat), backends=(0, 1))
+    elif cut == "per_type":
+        types = rng.integers(0, 2, nat).astype(np.int32)
+        _check(pos, cell, True, np.array([[4.0, 4.6], [4.6, 5.0]]), numbers=types,
+               backends=(0, 1))
+    else:
+        ref = _oracle("ijdDS", pos, cell, True, 5.0)
+        got = _engine("ijdDS", pos, cell, True, 5.0, f32=True)
+        for q in "ijS":
+            assert np.array_equal(got[q], ref[q])
+        np.testing.assert_allclose(got["d"], ref["d"], rtol=2e-6, atol=0)
+        scale = np.maximum(np.abs(ref["D"]), ref["d"][:, None])
+        assert np.all(np.abs(got["D"] - ref["D"]) <= 2e-6 * scale)
