@@ -159,9 +159,10 @@ def test_float32_forward_and_backward(gpu, bcc):
     i64, j64, D64, d64, S64 = neighbour_list_autograd("ijDdS", pos64, cell64, True, RC)
     # same pair set, D and d within 2e-6 relative of the float64 values
     assert torch.equal(i, i64) and torch.equal(j, j64) and torch.equal(S, S64)
-    assert float(((D.double() - D64).abs() / D64.abs().clamp_min(1e-30)).max()) <= 2e-6 or \
-        float((D.double() - D64).abs().max()) <= 2e-6 * float(D64.abs().max())
-    assert float(((d.double() - d64) / d64).abs().max()) <= 2e-6
+    Dd, D64d = D.detach().double(), D64.detach()
+    assert float(((Dd - D64d).abs() / D64d.abs().clamp_min(1e-30)).max()) <= 2e-6 or \
+        float((Dd - D64d).abs().max()) <= 2e-6 * float(D64d.abs().max())
+    assert float(((d.detach().double() - d64.detach()) / d64.detach()).abs().max()) <= 2e-6
     _loss(d.double()).backward()
     _loss(d64).backward()
     assert pos.grad.dtype == torch.float64
