@@ -167,6 +167,16 @@ PairArgs pair_args(const mnl_plan *p) {
     a.pt_sq = p->cut == CUT_PER_TYPE ? p->pt.as<double>() : nullptr;
     a.ncut = p->ncut;
     a.fill_chunks = p->fill_chunks;
+    /* Systems with few home cells (or slots) share each cell's atoms out
+       over several waves: about 4096 waves in all, at most one per atom
+       on average (fcc 1M: 97k cells, one wave each; 1000 atoms in 64
+       cells: 16 waves per cell). */
+    {
+        const int64_t cells = std::max<int64_t>(p->home_cells, 1);
+        const int64_t want = (4096 + cells - 1) / cells;
+        const int64_t per_cell = (static_cast<int64_t>(p->nat) + cells - 1) / cells;
+        a.split = static_cast<int32_t>(std::max<int64_t>(1, std::min(want, per_cell)));
+    }
     a.atom_base = p->atom_base.as<int32_t>();
     a.nat = p->nat;
     /* the count visits half the offsets when the test is symmetric in i, j */

@@ -87,6 +87,7 @@ struct PairArgs {
     int32_t ncut;
     int32_t xcd_blocks;        /* grid size, for the XCD-aware block remap */
     int32_t fill_chunks;       /* fill: candidate chunks per register window (4 or 5) */
+    int32_t split;             /* waves per home cell, each with a share of its atoms */
     const int32_t *atom_base;  /* [nsys+1] */
     int32_t nat;               /* atoms of all systems */
     int32_t half;              /* count: visit half the offsets (symmetric test) */

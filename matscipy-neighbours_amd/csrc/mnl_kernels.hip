@@ -652,6 +652,26 @@ __device__ __forceinline__ bool load_home(const PairArgs &A, int h, HomeCell &H)
     return true;
 }
 
+/* Wave w -> home cell w / split and part w % split. A cell's home atoms
+   are shared out over its split waves (each atom's row is independent), so
+   that systems with few cells still give the chip thousands of waves. */
+__device__ __forceinline__ void wave_home(const PairArgs &A, int w, int &h, int &part) {
+    h = A.split > 1 ? w / A.split : w;
+    part = w - h * A.split;
+}
+
+/* Home atoms [lo, hi) of part `part` of a cell with m atoms (split parts,
+   sizes differing by at most one). */
+__device__ __forceinline__ void home_part(int m, int part, int split, int &lo, int &hi) {
+    if (split == 1) {
+        lo = 0;
+        hi = m;
+        return;
+    }
+    lo = static_cast<int>(static_cast<int64_t>(m) * part / split);
+    hi = static_cast<int>(static_cast<int64_t>(m) * (part + 1) / split);
+}
+
 /* Segment table of one group of up to 64 neighbour-cell offsets, in the
    reference's traversal order. */
 struct SegLds {
@@ -808,13 +828,15 @@ __device__ __forceinline__ bool pair_test(const PairArgs &A, const HomeCell &H, 
  * count. Home atoms are taken in groups of 64.
  */
 template <int CUT, bool HASHED>
-__device__ __forceinline__ void count_cell(const PairArgs &A, int h, SegLds &S,
+__device__ __forceinline__ void count_cell(const PairArgs &A, int h, int part, SegLds &S,
                                            lds_dptr pt_sq) {
     HomeCell H;
     if (!load_home<HASHED>(A, h, H)) return;
     const int lane = lane_id();
-    for (int grp0 = 0; grp0 < H.m; grp0 += kWave) {
-        const int mg = uniform(min(kWave, H.m - grp0));
+    int lo, hi;
+    home_part(H.m, part, A.split, lo, hi);
+    for (int grp0 = lo; grp0 < hi; grp0 += kWave) {
+        const int mg = uniform(min(kWave, hi - grp0));
         int acc = 0;
         for (int sg = 0; sg < H.nseg; sg += kWave) {
             const int nsg = min(kWave, H.nseg - sg);
@@ -894,13 +916,15 @@ __device__ __forceinline__ void half_sweep(const PairArgs &A, const HomeCell &H,
 }
 
 template <int CUT, bool HASHED>
-__device__ __forceinline__ void count_cell_half(const PairArgs &A, int h, SegLds &S,
+__device__ __forceinline__ void count_cell_half(const PairArgs &A, int h, int part, SegLds &S,
                                                 lds_dptr pt_sq) {
     HomeCell H;
     if (!load_home<HASHED>(A, h, H)) return;
     const int lane = lane_id();
-    for (int grp0 = 0; grp0 < H.m; grp0 += kWave) {
-        const int mg = uniform(min(kWave, H.m - grp0));
+    int lo, hi;
+    home_part(H.m, part, A.split, lo, hi);
+    for (int grp0 = lo; grp0 < hi; grp0 += kWave) {
+        const int mg = uniform(min(kWave, hi - grp0));
         int acc = 0;
         for (int sg = H.centre; sg < H.nseg; sg += kWave) {
             const int nsg = min(kWave, H.nseg - sg);
@@ -956,12 +980,13 @@ __global__ __launch_bounds__(kBlock) void k_count(PairArgs A) {
     const lds_dptr pt_sq = stage_types<CUT>(A, s_pt);
     const int wib = uniform(threadIdx.x / kWave);
     const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
-    const int h = blk * kWavesPerBlock + wib;
+    int h, part;
+    wave_home(A, blk * kWavesPerBlock + wib, h, part);
     if (h >= A.home_cells) return;
     if (HALF)
-        count_cell_half<CUT, HASHED>(A, h, s_lds[wib], pt_sq);
+        count_cell_half<CUT, HASHED>(A, h, part, s_lds[wib], pt_sq);
     else
-        count_cell<CUT, HASHED>(A, h, s_lds[wib], pt_sq);
+        count_cell<CUT, HASHED>(A, h, part, s_lds[wib], pt_sq);
 }
 
 /* ------------------------------------------------------------- fill pass */
@@ -1069,13 +1094,15 @@ __device__ __forceinline__ void flush_ring(const PairArgs &A, const HomeCell &H,
  * ranks continue across windows, so an atom's row fills front to back.
  */
 template <int MODE, int CUT, bool HASHED, int FC>
-__device__ __forceinline__ void fill_cell(const PairArgs &A, int h, FillLds &L,
+__device__ __forceinline__ void fill_cell(const PairArgs &A, int h, int part, FillLds &L,
                                           lds_dptr pt_sq) {
     HomeCell H;
     if (!load_home<HASHED>(A, h, H)) return;
     const int lane = lane_id();
-    for (int grp0 = 0; grp0 < H.m; grp0 += kWave) {
-        const int mg = uniform(min(kWave, H.m - grp0));
+    int lo, hi;
+    home_part(H.m, part, A.split, lo, hi);
+    for (int grp0 = lo; grp0 < hi; grp0 += kWave) {
+        const int mg = uniform(min(kWave, hi - grp0));
         if (lane < mg) {
             const int4 inf = A.sinfo[H.hb + grp0 + lane];
             /* pair list: the atom's row offset; matrix: its row of K slots */
@@ -1163,9 +1190,10 @@ __global__ __launch_bounds__(kBlock) void k_fill(PairArgs A) {
     const lds_dptr pt_sq = stage_types<CUT>(A, s_pt);
     const int wib = uniform(threadIdx.x / kWave);
     const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
-    const int h = blk * kWavesPerBlock + wib;
+    int h, part;
+    wave_home(A, blk * kWavesPerBlock + wib, h, part);
     if (h >= A.home_cells) return;
-    fill_cell<MODE, CUT, HASHED, FC>(A, h, s_lds[wib], pt_sq);
+    fill_cell<MODE, CUT, HASHED, FC>(A, h, part, s_lds[wib], pt_sq);
 }
 
 /* Device-side parameter set-up (keeps the whole build free of host-to-device
@@ -1505,8 +1533,8 @@ static void launch_pairs_h(int mode, bool hashed, const PairArgs &a, unsigned nb
 
 hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
                         hipStream_t stream) {
-    /* one wave per home cell */
-    const int64_t waves = static_cast<int64_t>(args.home_cells);
+    /* split waves per home cell */
+    const int64_t waves = static_cast<int64_t>(args.home_cells) * std::max(args.split, 1);
     if (waves <= 0) return hipSuccess;
     const unsigned nb = static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock);
     PairArgs a = args;
