@@ -991,9 +991,9 @@ __global__ __launch_bounds__(kBlock) void k_count(PairArgs A) {
 
 /* ------------------------------------------------------------- fill pass */
 
-/* Candidate chunks held in registers while the home atoms are swept, and
-   the ring of accepted pairs waiting for a full 64-lane store. */
-/* (kFillChunks is the default; a fill launch may take 5, see launch_fill_t) */
+/* Candidate chunks held in registers while the home atoms are swept (the
+   default; a launch may take 5, see launch_fill_t), and the ring of
+   accepted pairs waiting for a full 64-lane store. */
 constexpr int kFillChunks = 4;
 constexpr int kRing = 128;
 
