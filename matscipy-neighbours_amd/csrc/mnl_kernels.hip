@@ -889,14 +889,14 @@ __device__ __forceinline__ int add_lane_bit(int acc, uint64_t mask) {
 }
 
 /* One candidate chunk against home atoms grp0 .. grp0+mg-1 (half count):
-   lane t of `part` gets home atom t's accepted candidates, cj the home
+   lane t of `icnt` gets home atom t's accepted candidates, cj the home
    atoms each candidate accepted. CENTRE: the chunk holds home-cell atoms
    (candidates c < m); home atom g takes only those with c > g, a scalar
    mask over the chunk's low lanes. */
 template <int CUT, bool CENTRE>
 __device__ __forceinline__ void half_sweep(const PairArgs &A, const HomeCell &H, const Cand &C,
                                            int grp0, int mg, int cb, lds_dptr pt_sq,
-                                           uint32_t &part, int &cj) {
+                                           uint32_t &icnt, int &cj) {
     for (int t = 0; t < mg; t++) {
         const int si = H.hb + grp0 + t;
         const double4 pi = load_const(A.spos, si);
@@ -911,7 +911,7 @@ __device__ __forceinline__ void half_sweep(const PairArgs &A, const HomeCell &H,
                 mask &= ~0ull << (hi + 1);
         }
         cj = add_lane_bit(cj, mask);
-        part = write_lane(part, static_cast<uint32_t>(__popcll(mask)), t);
+        icnt = write_lane(icnt, static_cast<uint32_t>(__popcll(mask)), t);
     }
 }
 
@@ -933,15 +933,15 @@ __device__ __forceinline__ void count_cell_half(const PairArgs &A, int h, int pa
             for (int cb = 0; cb < total; cb += kWave) {
                 Cand C;
                 load_candidate<CUT, false>(A, H, S, nsg, cb + lane, total, C);
-                uint32_t part = 0;
+                uint32_t icnt = 0;
                 int cj = 0;
                 /* the centre segment leads the first group: its candidates
                    c < m are the home atoms, taken only after the home atom */
                 if (sg == H.centre && cb < H.m)
-                    half_sweep<CUT, true>(A, H, C, grp0, mg, cb, pt_sq, part, cj);
+                    half_sweep<CUT, true>(A, H, C, grp0, mg, cb, pt_sq, icnt, cj);
                 else
-                    half_sweep<CUT, false>(A, H, C, grp0, mg, cb, pt_sq, part, cj);
-                acc += static_cast<int>(part);
+                    half_sweep<CUT, false>(A, H, C, grp0, mg, cb, pt_sq, icnt, cj);
+                acc += static_cast<int>(icnt);
                 if (cj) atomicAdd(A.scount + C.j, cj);
             }
             __builtin_amdgcn_wave_barrier();
