@@ -120,8 +120,8 @@ bool valid_quantities(int q) { return (q & ~31) == 0; }
 struct mnl_plan {
     int device = 0;
     /* workspaces */
-    DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, scount, counts,
-        order_stats,
+    DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, scount, slot,
+        counts, order_stats,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
         htypes, cell_partial, tri_m, tri_cnt, tri_off, tri_vidx, tri_flag;
     int64_t *host_total = nullptr; /* pinned */
@@ -213,6 +213,7 @@ int ensure_workspaces(mnl_plan *p, hipStream_t stream, int32_t nat, int64_t cell
                      ? sizeof(double) * std::max<int64_t>(1, int64_t(ncutoffs) * ncutoffs)
                      : 0},
         {&p->scount, n * sizeof(int32_t)},
+        {&p->slot, n * sizeof(int32_t)},
         {&p->counts, n * sizeof(int32_t)},
         {&p->offsets, (n + 1) * sizeof(int64_t)},
         {&p->cell_count, static_cast<size_t>(std::max<int64_t>(cells, 1)) * sizeof(int32_t)},
@@ -276,7 +277,7 @@ int enqueue_binning(mnl_plan *p, hipStream_t stream, const double *positions,
                                p->geo.as<SysGeom>(), p->atom_base.as<int32_t>(), p->nsys,
                                per_atom, types, p->spos.as<double4>(), p->sinfo.as<int4>(),
                                p->cut == CUT_PER_TYPE ? p->stype.as<int32_t>() : nullptr,
-                               p->scount.as<int32_t>(), stream));
+                               p->scount.as<int32_t>(), p->slot.as<int32_t>(), stream));
     return MNL_SUCCESS;
 }
 
@@ -294,9 +295,15 @@ int enqueue_build(mnl_plan *p, hipStream_t stream, const double *positions,
     const int32_t nat = p->nat;
     PairArgs a = pair_args(p);
     MNL_HIP(launch_pairs(0, p->cut, hashed, a, stream));
-    MNL_HIP(launch_scan_i32_to_i64(p->counts.as<int32_t>(), nat, p->offsets.as<int64_t>(),
-                                   p->partial.as<int64_t>(), p->total.as<int64_t>(), stream,
-                                   total_copy));
+    if (a.half)
+        MNL_HIP(launch_scan_sorted_counts(p->scount.as<int32_t>(), p->slot.as<int32_t>(),
+                                          p->counts.as<int32_t>(), nat,
+                                          p->offsets.as<int64_t>(), p->partial.as<int64_t>(),
+                                          p->total.as<int64_t>(), stream, total_copy));
+    else
+        MNL_HIP(launch_scan_i32_to_i64(p->counts.as<int32_t>(), nat, p->offsets.as<int64_t>(),
+                                       p->partial.as<int64_t>(), p->total.as<int64_t>(),
+                                       stream, total_copy));
     return MNL_SUCCESS;
 }
 
@@ -642,7 +649,7 @@ void mnl_plan_destroy(mnl_plan *p) {
     if (!p) return;
     DeviceScope ds(p->device);
     DevBuf *bufs[] = {&p->raw4,      &p->ticket,    &p->tmp,    &p->spos,     &p->sinfo,
-                      &p->stype,     &p->cell_count, &p->cell_first, &p->scount, &p->counts, &p->order_stats, &p->offsets,
+                      &p->stype,     &p->cell_count, &p->cell_first, &p->scount, &p->slot, &p->counts, &p->order_stats, &p->offsets,
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
                       &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
                       &p->cell_partial, &p->tri_m,   &p->tri_cnt,  &p->tri_off,

@@ -11,6 +11,7 @@
  *   stype  int32  [N]  cell-sorted atom type (per-type cutoffs only)
  *   cell_count / cell_first int32 [C] dense CSR (or hash slots when hashed)
  *   scount int32  [N]  neighbours per atom, cell-sorted order (half count)
+ *   slot   int32  [N]  cell-sorted index of each atom
  *   counts int32  [N]  neighbours per atom, original order
  *   offsets int64 [N+1] exclusive scan of counts (CSR row pointers)
  * A "system" is one periodic cell with its atoms; the plain path is a batch
@@ -125,7 +126,7 @@ hipError_t launch_rank_gather(const double *r, const int4 *raw4, const int32_t *
                               const int32_t *atom_base, int32_t nsys,
                               const double *per_atom, const int32_t *types,
                               double4 *spos, int4 *sinfo, int32_t *stype,
-                              int32_t *scount, hipStream_t stream);
+                              int32_t *scount, int32_t *slot, hipStream_t stream);
 /* Exclusive scan of n int32 values; out gets n (+1 with the total) entries.
    partial needs scan_partial_size(n) int64 entries. total and total2
    (device, may be NULL) receive the sum. */
@@ -136,6 +137,12 @@ hipError_t launch_scan_i32_to_i32(const int32_t *in, int64_t n, int32_t *out,
 hipError_t launch_scan_i32_to_i64(const int32_t *in, int64_t n, int64_t *out,
                                   int64_t *partial, int64_t *total,
                                   hipStream_t stream, int64_t *total2 = nullptr);
+/* The same scan over per-atom counts held in cell-sorted order (the half
+   count): value of atom a = scount[slot[a]], also written to counts[a]. */
+hipError_t launch_scan_sorted_counts(const int32_t *scount, const int32_t *slot,
+                                     int32_t *counts, int64_t n, int64_t *out,
+                                     int64_t *partial, int64_t *total, hipStream_t stream,
+                                     int64_t *total2 = nullptr);
 hipError_t launch_scan_i64_to_i64(const int64_t *in, int64_t n, int64_t *out,
                                   int64_t *partial, int64_t *total,
                                   hipStream_t stream);

@@ -315,7 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
     const int32_t *__restrict__ atom_base, int32_t nsys,
     const double *__restrict__ per_atom, const int32_t *__restrict__ types,
     double4 *__restrict__ spos, int4 *__restrict__ sinfo, int32_t *__restrict__ stype,
-    int32_t *__restrict__ scount) {
+    int32_t *__restrict__ scount, int32_t *__restrict__ slot) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nat) return;
     const int4 rc = raw4[a];
@@ -359,6 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
     sinfo[s] = make_int4(base[0], base[1], base[2], a - g.atom_base);
     if (stype) stype[s] = types ? types[a] : 0;
     if (scount) scount[s] = 0; /* the half count accumulates here */
+    if (slot) slot[a] = s;     /* atom -> cell-sorted index */
 }
 
 /* ---------------------------------------------------------------- scan */
@@ -374,6 +375,36 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const TI *__restrict__ i
     for (int k = 0; k < kScanItems; k++) {
         const int64_t idx = base + k * kBlock + threadIdx.x;
         if (idx < n) v += in[idx];
+    }
+    v = wave_sum(v);
+    if (lane_id() == 0) s_w[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int w = 0; w < kWavesPerBlock; w++) t += s_w[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+/* k_scan_reduce over counts held in cell-sorted order: the value of atom a
+   is scount[slot[a]], also written to counts[a] (the half count's per-atom
+   counts in atom order, read by the apply phase). */
+__global__ __launch_bounds__(kBlock) void k_scan_reduce_gather(const int32_t *__restrict__ scount,
+                                                                const int32_t *__restrict__ slot,
+                                                                int32_t *__restrict__ counts,
+                                                                int64_t n,
+                                                                int64_t *__restrict__ partial) {
+    __shared__ int64_t s_w[kWavesPerBlock];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+    int64_t v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int64_t idx = base + k * kBlock + threadIdx.x;
+        if (idx < n) {
+            const int32_t c = scount[slot[idx]];
+            counts[idx] = c;
+            v += c;
+        }
     }
     v = wave_sum(v);
     if (lane_id() == 0) s_w[threadIdx.x / kWave] = v;
@@ -950,15 +981,6 @@ __device__ __forceinline__ void count_cell_half(const PairArgs &A, int h, int pa
     }
 }
 
-/* Half count -> counts by global atom index. A system's atoms occupy the
-   same index range in cell-sorted and in original order. */
-__global__ __launch_bounds__(kBlock) void k_unsort_counts(PairArgs A) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= A.nat) return;
-    const int sys = system_of(A.atom_base, A.nsys, s);
-    A.counts[A.atom_base[sys] + A.sinfo[s].w] = A.scount[s];
-}
-
 /* Per-type cutoff tables up to this many entries are staged in LDS once per
    block (the test reads one entry per candidate and home atom); larger ones
    are read from global memory. */
@@ -1427,13 +1449,13 @@ hipError_t launch_rank_gather(const double *r, const int4 *raw4, const int32_t *
                               const int32_t *cell_count, const SysGeom *geo,
                               const int32_t *atom_base, int32_t nsys,
                               const double *per_atom, const int32_t *types, double4 *spos,
-                              int4 *sinfo, int32_t *stype, int32_t *scount,
+                              int4 *sinfo, int32_t *stype, int32_t *scount, int32_t *slot,
                               hipStream_t stream) {
     if (nat <= 0) return hipSuccess;
     k_rank_gather<<<grid_for(nat), kBlock, 0, stream>>>(r, raw4, tmp, nat, cell_first,
                                                         cell_count, geo, atom_base, nsys,
                                                         per_atom, types, spos, sinfo, stype,
-                                                        scount);
+                                                        scount, slot);
     return hipGetLastError();
 }
 
@@ -1442,7 +1464,8 @@ int64_t scan_partial_size(int64_t n) { return (n + kScanTile - 1) / kScanTile + 
 template <typename TI, typename TO>
 static hipError_t scan_impl(const TI *in, int64_t n, TO *out, int64_t *partial,
                             int64_t *total, int64_t *total2, bool write_total,
-                            hipStream_t stream) {
+                            hipStream_t stream, const int32_t *gsrc = nullptr,
+                            const int32_t *gslot = nullptr) {
     if (n <= 0) {
         if (write_total) {
             /* out[0] = 0 */
@@ -1456,7 +1479,11 @@ static hipError_t scan_impl(const TI *in, int64_t n, TO *out, int64_t *partial,
         return hipGetLastError();
     }
     const int64_t nb = (n + kScanTile - 1) / kScanTile;
-    k_scan_reduce<TI><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(in, n, partial);
+    if (gslot)
+        k_scan_reduce_gather<<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(
+            gsrc, gslot, reinterpret_cast<int32_t *>(const_cast<TI *>(in)), n, partial);
+    else
+        k_scan_reduce<TI><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(in, n, partial);
     /* up to this many tiles, each apply block adds up the tile sums before
        it (no spine kernel) */
     constexpr int64_t kFusedTiles = 1024;
@@ -1479,6 +1506,14 @@ hipError_t launch_scan_i32_to_i64(const int32_t *in, int64_t n, int64_t *out,
     return scan_impl<int32_t, int64_t>(in, n, out, partial, total, total2, true, stream);
 }
 
+hipError_t launch_scan_sorted_counts(const int32_t *scount, const int32_t *slot,
+                                     int32_t *counts, int64_t n, int64_t *out,
+                                     int64_t *partial, int64_t *total, hipStream_t stream,
+                                     int64_t *total2) {
+    return scan_impl<int32_t, int64_t>(counts, n, out, partial, total, total2, true, stream,
+                                       scount, slot);
+}
+
 hipError_t launch_scan_i64_to_i64(const int64_t *in, int64_t n, int64_t *out,
                                   int64_t *partial, int64_t *total, hipStream_t stream) {
     return scan_impl<int64_t, int64_t>(in, n, out, partial, total, nullptr, true, stream);
@@ -1487,8 +1522,9 @@ hipError_t launch_scan_i64_to_i64(const int64_t *in, int64_t n, int64_t *out,
 template <int CUT, bool HASHED>
 static void launch_count_t(const PairArgs &a, unsigned nblocks, hipStream_t stream) {
     if (a.half) {
+        /* per-atom counts stay in cell-sorted order: the caller scans them
+           with launch_scan_sorted_counts */
         k_count<CUT, HASHED, true><<<nblocks, kBlock, 0, stream>>>(a);
-        if (a.nat > 0) k_unsort_counts<<<grid_for(a.nat), kBlock, 0, stream>>>(a);
     } else {
         k_count<CUT, HASHED, false><<<nblocks, kBlock, 0, stream>>>(a);
     }
