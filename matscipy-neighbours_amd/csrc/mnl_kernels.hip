@@ -995,8 +995,12 @@ __device__ __forceinline__ lds_dptr stage_types(const PairArgs &A, double *s_pt)
     return (lds_dptr)(s_pt);
 }
 
+/* 8 waves per SIMD (the scalar registers would otherwise allow 7; the
+   excess spills to vector lanes): the count is latency-bound in its per-cell
+   set-up, and the extra wave hides some of it. */
 template <int CUT, bool HASHED, bool HALF>
-__global__ __launch_bounds__(kBlock) void k_count(PairArgs A) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_count(
+    PairArgs A) {
     __shared__ SegLds s_lds[kWavesPerBlock];
     __shared__ double s_pt[CUT == CUT_PER_TYPE_LDS ? kLdsTypes : 1];
     const lds_dptr pt_sq = stage_types<CUT>(A, s_pt);
