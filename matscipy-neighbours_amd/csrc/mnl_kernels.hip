@@ -1277,32 +1277,55 @@ __device__ __forceinline__ double seg_incl_sum(double v, int key) {
     return v;
 }
 
+/* Per-block table of j-side sums (LDS): slots, and linear probes tried
+   before a pair adds straight to global memory. */
+constexpr int kHashSlots = 1024;
+constexpr int kHashProbe = 8;
+
 /*
  * Backward of D_p = r[j] - r[i] + S_p . cell and d_p = |D_p| over an
  * i-sorted pair list. Per pair g = dL/dD + (dL/dd) D / d (the d term is
- * dropped for coincident atoms, d = 0). dL/dr[j] += g by f64 atomics;
- * dL/dr[i] -= g summed first over each i-segment inside the wave (one
- * atomic per segment per 64 pairs); dL/dcell[a][b] = sum_p S_pa g_pb kept
- * per thread over a grid-stride loop, reduced per block into
- * cell_partial[block][9] (fixed grid, fixed order: deterministic).
+ * dropped for coincident atoms, d = 0). Each block takes a contiguous tile
+ * of pairs:
+ *   dL/dr[j] += g goes through a per-block LDS table keyed by j (open
+ *     addressing, CAS on the key, LDS f64 adds): the rows of a tile are
+ *     consecutive atoms whose neighbours overlap, so the table holds far
+ *     fewer atoms than the tile has pairs, and it is flushed with one global
+ *     atomic per atom and component; a pair whose probes all fail adds
+ *     straight to global memory;
+ *   dL/dr[i] -= g summed first over each i-segment inside the wave (one
+ *     atomic per segment per 64 pairs);
+ *   dL/dcell[a][b] = sum_p S_pa g_pb kept per thread, reduced per block into
+ *     cell_partial[block][9] (fixed tiles, fixed order: deterministic).
+ * The position sums go to separate x, y, z arrays (gx, gy, gz).
  */
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pair_backward(
-    int64_t npairs, const int32_t *__restrict__ first, const int32_t *__restrict__ secnd,
-    const int32_t *__restrict__ shift, const T *__restrict__ D, const T *__restrict__ d,
-    const T *__restrict__ gD, const T *__restrict__ gd, double *__restrict__ gx,
-    double *__restrict__ gy, double *__restrict__ gz, double *__restrict__ cell_partial) {
+    int64_t npairs, int64_t tile, const int32_t *__restrict__ first,
+    const int32_t *__restrict__ secnd, const int32_t *__restrict__ shift,
+    const T *__restrict__ D, const T *__restrict__ d, const T *__restrict__ gD,
+    const T *__restrict__ gd, double *__restrict__ gx, double *__restrict__ gy,
+    double *__restrict__ gz, double *__restrict__ cell_partial) {
     __shared__ double s_red[kWavesPerBlock][9];
+    __shared__ int s_key[kHashSlots];
+    __shared__ double s_g[3][kHashSlots];
+    for (int k = threadIdx.x; k < kHashSlots; k += kBlock) {
+        s_key[k] = -1;
+        s_g[0][k] = 0.0;
+        s_g[1][k] = 0.0;
+        s_g[2][k] = 0.0;
+    }
+    __syncthreads();
     double c[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) c[k] = 0.0;
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
-    /* every lane runs the same trip count so that the wave shuffles stay
-       converged; lanes past the end carry key -1 and zeros */
-    const int64_t base0 = static_cast<int64_t>(blockIdx.x) * kBlock + (threadIdx.x & ~(kWave - 1));
-    for (int64_t wb = base0; wb < npairs; wb += stride) {
+    const int64_t t0 = static_cast<int64_t>(blockIdx.x) * tile;
+    const int64_t t1 = t0 + tile < npairs ? t0 + tile : npairs;
+    /* every lane of a wave runs the same trip count so that the shuffles
+       stay converged; lanes past the tile carry key -1 and zeros */
+    for (int64_t wb = t0 + (threadIdx.x & ~(kWave - 1)); wb < t1; wb += kBlock) {
         const int64_t p = wb + lane_id();
-        const bool valid = p < npairs;
+        const bool valid = p < t1;
         int i = -1, j = -1;
         double g0 = 0.0, g1 = 0.0, g2 = 0.0;
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
@@ -1326,9 +1349,26 @@ __global__ __launch_bounds__(kBlock) void k_pair_backward(
             s0 = static_cast<double>(shift[3 * p + 0]);
             s1 = static_cast<double>(shift[3 * p + 1]);
             s2 = static_cast<double>(shift[3 * p + 2]);
-            unsafeAtomicAdd(&gx[j], g0);
-            unsafeAtomicAdd(&gy[j], g1);
-            unsafeAtomicAdd(&gz[j], g2);
+            /* j-side: the block's table, or global memory when full */
+            const unsigned hsh = static_cast<unsigned>(j) * 2654435761u;
+            int slot = -1;
+            for (int q = 0; q < kHashProbe; q++) {
+                const int k = static_cast<int>((hsh >> 22) + q) & (kHashSlots - 1);
+                const int old = atomicCAS(&s_key[k], -1, j);
+                if (old == -1 || old == j) {
+                    slot = k;
+                    break;
+                }
+            }
+            if (slot >= 0) {
+                atomicAdd(&s_g[0][slot], g0);
+                atomicAdd(&s_g[1][slot], g1);
+                atomicAdd(&s_g[2][slot], g2);
+            } else {
+                unsafeAtomicAdd(&gx[j], g0);
+                unsafeAtomicAdd(&gy[j], g1);
+                unsafeAtomicAdd(&gz[j], g2);
+            }
         }
         c[0] += s0 * g0; c[1] += s0 * g1; c[2] += s0 * g2;
         c[3] += s1 * g0; c[4] += s1 * g1; c[5] += s1 * g2;
@@ -1342,6 +1382,15 @@ __global__ __launch_bounds__(kBlock) void k_pair_backward(
             unsafeAtomicAdd(&gx[i], -a0);
             unsafeAtomicAdd(&gy[i], -a1);
             unsafeAtomicAdd(&gz[i], -a2);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kHashSlots; k += kBlock) {
+        const int jk = s_key[k];
+        if (jk >= 0) {
+            unsafeAtomicAdd(&gx[jk], s_g[0][k]);
+            unsafeAtomicAdd(&gy[jk], s_g[1][k]);
+            unsafeAtomicAdd(&gz[jk], s_g[2][k]);
         }
     }
 #pragma unroll
@@ -1607,9 +1656,16 @@ hipError_t launch_system_offsets(const int64_t *offsets, const int32_t *atom_bas
     return hipGetLastError();
 }
 
+/* Tiles of at least 8192 pairs (a block's table then collects many pairs
+   per atom), at most kBackwardMaxBlocks of them. */
 int backward_blocks(int64_t npairs) {
-    const int64_t want = (npairs + kBlock - 1) / kBlock;
+    const int64_t want = (npairs + 8191) / 8192;
     return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, kBackwardMaxBlocks)));
+}
+
+static int64_t backward_tile(int64_t npairs, int nb) {
+    const int64_t t = (npairs + nb - 1) / nb;
+    return (t + kWave - 1) / kWave * kWave;
 }
 
 hipError_t launch_pair_backward(int64_t npairs, int32_t nat, const int32_t *first,
@@ -1625,13 +1681,13 @@ hipError_t launch_pair_backward(int64_t npairs, int32_t nat, const int32_t *firs
     if (npairs > 0) {
         if (f32)
             k_pair_backward<float><<<nb, kBlock, 0, stream>>>(
-                npairs, first, secnd, shift, static_cast<const float *>(D),
+                npairs, backward_tile(npairs, nb), first, secnd, shift, static_cast<const float *>(D),
                 static_cast<const float *>(d), static_cast<const float *>(gD),
                 static_cast<const float *>(gd), soa, soa + nat, soa + 2 * int64_t(nat),
                 cell_partial);
         else
             k_pair_backward<double><<<nb, kBlock, 0, stream>>>(
-                npairs, first, secnd, shift, static_cast<const double *>(D),
+                npairs, backward_tile(npairs, nb), first, secnd, shift, static_cast<const double *>(D),
                 static_cast<const double *>(d), static_cast<const double *>(gD),
                 static_cast<const double *>(gd), soa, soa + nat, soa + 2 * int64_t(nat),
                 cell_partial);
