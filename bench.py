@@ -334,6 +334,16 @@ def other_workloads(lib, _lib, torch, dev, steps, warmup, with_cpu):
         "hbm_frac": b / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "backend": "dense" if lib.mnl_plan_backend(step.plan.handle) == 0 else "hashed"}
 
+    # the headline fcc workload with the float32 output option (D, d float32;
+    # pairs still decided in float64): 36 B per pair instead of 52
+    w = workloads.fcc_bulk()
+    step, P = _plan_step_fn(lib, _lib, torch, dev, w, f32=True)
+    ms = _time_steps(torch, stream, step, steps, warmup)
+    res["fcc_1M_ijdDS_f32"] = {
+        "atoms": step.N, "pairs": P, "ms_per_step": ms, "pairs_per_s": P / (ms * 1e-3),
+        "how": "count + fill of i, j, D (float32), d (float32), S into pre-allocated outputs"}
+    del step
+
     # differentiable bcc: forward ijdD (f64, f32) and the backward kernel
     w = workloads.bcc_sheared()
     fwd = {}
