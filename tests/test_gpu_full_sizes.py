@@ -112,3 +112,102 @@ def test_many_atoms_row_offsets(gpu):
     d = out["d"].cpu().numpy()[sel]
     assert np.max(np.abs(d - np.linalg.norm(D, axis=1))) < 1e-12
     assert np.all(d < 3.0)
+
+
+def test_beyond_int32_pairs(gpu):
+    """More pairs than an int32 can index (the reference's index_t limits
+    its lists to 2^31 - 1, src/libneighbours/types.hh:28; this engine's
+    counts, row offsets and output positions are int64): 26 M atoms of a
+    uniform gas at 0.1 atoms/A^3, rc 6 A, about 2.35 G pairs and 122 GB of
+    float64 ijdDS output. Checked on the device:
+
+    * the total exceeds 2^31 and equals the sum of the per-atom counts;
+    * i is sorted and every row starts at the prefix sum of the counts before
+      it and ends where the next starts (so every row holds its own atom's
+      pairs, including the rows written past position 2^31);
+    * j is a valid atom, S is in {-1, 0, 1} (rc is far below half the box);
+    * on 1 M random pairs plus the pairs around positions 2^31 and at the
+      end, D == r[j] - r[i] + S L and d == |D| < rc;
+    * 64 rows (48 of them past position 2^31, 16 of those of atoms near a
+      face, whose pairs cross the boundary) hold exactly the (j, S) that a
+      brute-force minimum-image search finds within rc, up to pairs within
+      1e-9 A of rc, whose membership the two roundings may decide apart.
+
+    first_neighbours, whose seeds are int32 as the reference's are, refuses
+    the list."""
+    import torch
+
+    from matscipy_neighbours_amd import neighbours as nb
+    free, total = torch.cuda.mem_get_info()
+    if free < 160 * 2**30:
+        pytest.skip(f"needs 160 GiB of free device memory, {free / 2**30:.0f} GiB free")
+    n = 26_000_000
+    rc = 6.0
+    L = (n / 0.1) ** (1.0 / 3.0)
+    rng = np.random.default_rng(2024)
+    r_host = rng.uniform(0.0, L, (n, 3))
+    cell = L * np.eye(3)
+    inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+    print(f"{n} atoms, box {L:.1f} A, rc {rc}", flush=True)
+    r = torch.from_numpy(r_host).to("cuda:0")
+    del r_host
+    b = nb._Build(r, np.zeros(3), cell, inv, np.ones(3, bool), rc, None)
+    out = b.run(nb._quantity_flags("ijdDS"), want_counts=True)
+    torch.cuda.synchronize()
+    i, j, D, d, S, c = (out[k] for k in ("i", "j", "D", "d", "S", "c"))
+    P = i.numel()
+    print(f"{P} pairs", flush=True)
+    assert P > 2**31 + 10**8
+    assert j.numel() == d.numel() == P and D.shape == S.shape == (P, 3)
+
+    off = torch.zeros(n + 1, dtype=torch.int64, device=r.device)
+    off[1:] = torch.cumsum(c.to(torch.int64), 0)
+    assert int(off[-1]) == P
+    assert bool((i[1:] >= i[:-1]).all())
+    full = torch.nonzero(c > 0).squeeze(1)
+    assert bool((i[off[full]] == full.to(torch.int32)).all())
+    assert bool((i[off[full + 1] - 1] == full.to(torch.int32)).all())
+    assert bool((j >= 0).all()) and bool((j < n).all())
+    assert bool((S >= -1).all()) and bool((S <= 1).all())
+    print("rows and ranges checked", flush=True)
+
+    g = torch.Generator(device=r.device).manual_seed(5)
+    sel = torch.cat([
+        torch.randint(0, P, (1_000_000,), generator=g, device=r.device),
+        torch.arange(2**31 - 50_000, 2**31 + 50_000, device=r.device),
+        torch.arange(P - 50_000, P, device=r.device)])
+    Ds = D[sel]
+    ref_D = r[j[sel].long()] - r[i[sel].long()] + S[sel].double() * L
+    assert float((Ds - ref_D).abs().max()) < 1e-9
+    ds = d[sel]
+    assert float((ds - torch.linalg.norm(Ds, dim=1)).abs().max()) < 1e-12
+    assert bool((ds < rc).all())
+    del Ds, ref_D, ds
+    print("sampled pairs checked", flush=True)
+
+    # rows past position 2^31, a third of them of atoms within rc of a face
+    # (the only ones with nonzero shifts), and rows anywhere
+    late = full[off[full] > 2**31]
+    edge = torch.minimum(r, L - r).min(dim=1).values < rc
+    late_edge = late[edge[late]]
+    pick = lambda src, k: src[torch.randint(0, src.numel(), (k,), generator=g,
+                                            device=r.device)]
+    rows = torch.cat([pick(late, 32), pick(late_edge, 16), pick(full, 16)])
+    for a in rows.tolist():
+        dr = r - r[a]
+        s = -torch.round(dr / L)
+        dist = torch.linalg.norm(dr + s * L, dim=1)
+        dist[a] = float("inf")
+        key = lambda jj, ss: (jj.long() * 4 + (ss[:, 0].long() + 1)) * 16 + \
+            (ss[:, 1].long() + 1) * 4 + (ss[:, 2].long() + 1)
+        lo, hi = int(off[a]), int(off[a + 1])
+        got = set(key(j[lo:hi], S[lo:hi]).tolist())
+        inner = torch.nonzero(dist < rc - 1e-9).squeeze(1)
+        outer = torch.nonzero(dist < rc + 1e-9).squeeze(1)
+        want_in = set(key(inner, s[inner]).tolist())
+        want_out = set(key(outer, s[outer]).tolist())
+        assert want_in <= got <= want_out, f"row {a}"
+    print("brute-force rows checked", flush=True)
+
+    with pytest.raises(RuntimeError, match="2\\^31"):
+        nb.first_neighbours(n, i)
