@@ -275,19 +275,14 @@ def _time_steps(torch, stream, step, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
-def other_workloads(lib, _lib, torch, dev, steps, warmup, with_cpu):
-    """The smaller workloads: the 1000-atom latency case, the per-type slab,
-    and the differentiable bcc case (forward f64 / f32 and backward)."""
+def leg_latency(lib, _lib, torch, dev, steps, warmup, with_cpu):
+    """1000 atoms, 10 A box, rc 2.5: latency of one full build (count, host
+    read of the pair total, fill) on the host clock."""
     import ctypes
 
     import numpy as np
 
     from matscipy_neighbours_amd import workloads
-    stream = torch.cuda.current_stream(dev)
-    res = {}
-
-    # 1000 atoms, 10 A box, rc 2.5: latency of one full build (count, host
-    # read of the pair total, fill) on the host clock
     w = workloads.uniform_box()
     step, P = _plan_step_fn(lib, _lib, torch, dev, w)
     cell = np.ascontiguousarray(w["cell"])
@@ -321,30 +316,41 @@ def other_workloads(lib, _lib, torch, dev, steps, warmup, with_cpu):
                                                     w["pbc"], w["positions"], 2.5, 51)
             entry["cpu_reference_us"] = float(np.median(secs[1:])) * 1e6
             entry["cpu_threads"] = int(threads)
-    res["uniform_1000_rc2.5"] = entry
+    return "uniform_1000_rc2.5", entry
 
-    # the slab: 200,000 atoms, two species, per-type cutoffs, pbc TTF
+
+def leg_slab(lib, _lib, torch, dev, steps, warmup, with_cpu):
+    """The slab: 200,000 atoms, two species, per-type cutoffs, pbc TTF."""
+    from matscipy_neighbours_amd import workloads
+    stream = torch.cuda.current_stream(dev)
     w = workloads.triclinic_slab()
     step, P = _plan_step_fn(lib, _lib, torch, dev, w)
     ms = _time_steps(torch, stream, step, steps, warmup)
     N = step.N
     b = 24 * N + 4 * N + 52 * P
-    res["slab_200k_pertype"] = {
+    return "slab_200k_pertype", {
         "atoms": N, "pairs": P, "ms_per_step": ms, "pairs_per_s": P / (ms * 1e-3),
         "hbm_frac": b / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "backend": "dense" if lib.mnl_plan_backend(step.plan.handle) == 0 else "hashed"}
 
-    # the headline fcc workload with the float32 output option (D, d float32;
-    # pairs still decided in float64): 36 B per pair instead of 52
+
+def leg_fcc_f32(lib, _lib, torch, dev, steps, warmup, with_cpu):
+    """The headline fcc workload with the float32 output option (D, d
+    float32; pairs still decided in float64): 36 B per pair instead of 52."""
+    from matscipy_neighbours_amd import workloads
+    stream = torch.cuda.current_stream(dev)
     w = workloads.fcc_bulk()
     step, P = _plan_step_fn(lib, _lib, torch, dev, w, f32=True)
     ms = _time_steps(torch, stream, step, steps, warmup)
-    res["fcc_1M_ijdDS_f32"] = {
+    return "fcc_1M_ijdDS_f32", {
         "atoms": step.N, "pairs": P, "ms_per_step": ms, "pairs_per_s": P / (ms * 1e-3),
         "how": "count + fill of i, j, D (float32), d (float32), S into pre-allocated outputs"}
-    del step
 
-    # differentiable bcc: forward ijdD (f64, f32) and the backward kernel
+
+def leg_autograd(lib, _lib, torch, dev, steps, warmup, with_cpu):
+    """Differentiable bcc: forward ijdD (f64, f32) and the backward kernel."""
+    from matscipy_neighbours_amd import workloads
+    stream = torch.cuda.current_stream(dev)
     w = workloads.bcc_sheared()
     fwd = {}
     for f32 in (False, True):
@@ -369,7 +375,7 @@ def other_workloads(lib, _lib, torch, dev, steps, warmup, with_cpu):
                                               _lib.ptr(o["D"]), _lib.ptr(o["d"]), _lib.ptr(gD),
                                               _lib.ptr(gd), 0, _lib.ptr(gpos), _lib.ptr(gcell)))
     ms_b = _time_steps(torch, stream, bwd, steps, warmup)
-    res["bcc_131k_autograd"] = {
+    return "bcc_131k_autograd", {
         "atoms": N, "pairs": P,
         "forward_ijdD_f64_ms": fwd["f64"][0], "forward_ijdD_f32_ms": fwd["f32"][0],
         "backward_ms": ms_b,
@@ -378,6 +384,20 @@ def other_workloads(lib, _lib, torch, dev, steps, warmup, with_cpu):
         "backward_hbm_frac": (108 * P + 24 * N) / (ms_b * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "how": "forward = count + fill of i, j, D, d; backward reads i, j, S, D, d, dL/dD, "
                "dL/dd (108 B/pair) and writes dL/dr (24 B/atom) and dL/dcell"}
+
+
+OTHER_LEGS = {"latency": leg_latency, "slab": leg_slab, "fcc_f32": leg_fcc_f32,
+              "autograd": leg_autograd}
+
+
+def other_workloads(lib, _lib, torch, dev, steps, warmup, with_cpu):
+    """The smaller workloads: the 1000-atom latency case, the per-type slab,
+    the float32 output option on fcc 1M, and the differentiable bcc case
+    (forward f64 / f32 and backward)."""
+    res = {}
+    for leg in OTHER_LEGS.values():
+        name, entry = leg(lib, _lib, torch, dev, steps, warmup, with_cpu)
+        res[name] = entry
     return res
 
 
@@ -390,8 +410,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-systems leg")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the other workloads (1000-atom latency, slab, bcc autograd)")
+                    help="skip the other workloads (1000-atom latency, slab, fcc 1M with "
+                         "float32 outputs, bcc autograd)")
     ap.add_argument("--reps", type=int, default=64, help="fcc unit cells per edge")
+    ap.add_argument("--leg", choices=["batch"] + list(OTHER_LEGS),
+                    help="run only this secondary workload (for profiling it on its own) "
+                         "and print its result as one JSON line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (to scale HBM "
                          "traffic counters) and exit")
@@ -440,6 +464,17 @@ def main():
     _lib.require_gpu()
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+
+    if args.leg:
+        _log(f"leg {args.leg}")
+        if args.leg == "batch":
+            name, entry = "batched", batched_leg(lib, _lib, torch, dev, args.steps, args.warmup)
+        else:
+            name, entry = OTHER_LEGS[args.leg](lib, _lib, torch, dev, args.steps, args.warmup,
+                                               not args.no_cpu)
+        print(json.dumps({"leg": name, "steps": args.steps, "warmup": args.warmup,
+                          "result": entry}), flush=True)
+        return
 
     _log("generating workload")
     w = workloads.fcc_bulk(reps=args.reps)
