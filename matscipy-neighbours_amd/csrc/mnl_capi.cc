@@ -286,11 +286,7 @@ int enqueue_binning(mnl_plan *p, hipStream_t stream, const double *positions,
  * if given, in *total_copy (device); does not synchronise and records no
  * memset, so it can be captured in a graph.
  */
-int enqueue_build(mnl_plan *p, hipStream_t stream, const double *positions,
-                  const double *per_atom, const int32_t *types,
-                  int64_t *total_copy = nullptr, bool cleared = false) {
-    if (enqueue_binning(p, stream, positions, per_atom, types, cleared) != MNL_SUCCESS)
-        return MNL_ERROR;
+int enqueue_count_scan(mnl_plan *p, hipStream_t stream, int64_t *total_copy) {
     const bool hashed = p->backend == MNL_BACKEND_HASHED;
     const int32_t nat = p->nat;
     PairArgs a = pair_args(p);
@@ -305,6 +301,14 @@ int enqueue_build(mnl_plan *p, hipStream_t stream, const double *positions,
                                        p->partial.as<int64_t>(), p->total.as<int64_t>(),
                                        stream, total_copy));
     return MNL_SUCCESS;
+}
+
+int enqueue_build(mnl_plan *p, hipStream_t stream, const double *positions,
+                  const double *per_atom, const int32_t *types,
+                  int64_t *total_copy = nullptr, bool cleared = false) {
+    if (enqueue_binning(p, stream, positions, per_atom, types, cleared) != MNL_SUCCESS)
+        return MNL_ERROR;
+    return enqueue_count_scan(p, stream, total_copy);
 }
 
 /*
@@ -371,14 +375,17 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
         MNL_SUCCESS)
         return MNL_ERROR;
 
+    /* a small dense system is binned by one block, parameters included */
+    const bool small = !hashed && small_binning_fits(nat, ncells);
     /* parameters are written on the device by the kernel that clears the
        grid: no host-to-device copy, nothing read from host memory later */
-    MNL_HIP(launch_set_params(p->geo.as<SysGeom>(), g, p->atom_base.as<int32_t>(),
-                              p->cell_base.as<int32_t>(), nat,
-                              static_cast<int32_t>(hashed ? 0 : ncells),
-                              p->cell_count.as<int32_t>(), cells_or_slots,
-                              hashed ? p->hkey.as<int64_t>() : nullptr,
-                              hashed ? nullptr : p->order_stats.as<int32_t>(), stream));
+    if (!small)
+        MNL_HIP(launch_set_params(p->geo.as<SysGeom>(), g, p->atom_base.as<int32_t>(),
+                                  p->cell_base.as<int32_t>(), nat,
+                                  static_cast<int32_t>(hashed ? 0 : ncells),
+                                  p->cell_count.as<int32_t>(), cells_or_slots,
+                                  hashed ? p->hkey.as<int64_t>() : nullptr,
+                                  hashed ? nullptr : p->order_stats.as<int32_t>(), stream));
     if (p->cut == CUT_PER_TYPE && ncutoffs > 0) {
         /* the table is uploaded only when it differs from the plan's copy, so
            steady-state builds with fixed cutoffs stay asynchronous */
@@ -401,6 +408,18 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
                                    hipMemcpyHostToDevice, stream));
             MNL_HIP(hipStreamSynchronize(stream));
         }
+    }
+    if (small) {
+        MNL_HIP(launch_bin_small(positions, nat, g, static_cast<int32_t>(ncells),
+                                 p->geo.as<SysGeom>(), p->atom_base.as<int32_t>(),
+                                 p->cell_base.as<int32_t>(), p->order_stats.as<int32_t>(),
+                                 per_atom, types, p->cell_first.as<int32_t>(),
+                                 p->cell_count.as<int32_t>(), p->spos.as<double4>(),
+                                 p->sinfo.as<int4>(),
+                                 p->cut == CUT_PER_TYPE ? p->stype.as<int32_t>() : nullptr,
+                                 p->scount.as<int32_t>(), p->slot.as<int32_t>(), stream));
+        if (!with_count) return MNL_SUCCESS;
+        return enqueue_count_scan(p, stream, total_copy);
     }
     if (!with_count) return enqueue_binning(p, stream, positions, per_atom, types, true);
     return enqueue_build(p, stream, positions, per_atom, types, total_copy, true);

@@ -169,6 +169,18 @@ hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
    cell_count[0..nslots) = 0 and, if hkey is given, hkey[0..nslots) = -1.
    order_stats (may be NULL): the previous binning's per-axis counts, which
    choose the dense layout, then cleared. */
+/* The whole binning of one small dense system (at most 4096 atoms and 4096
+   cells: small_binning_fits) in a single block: writes the parameters (with
+   the layout chosen from order_stats) and everything set_params, bin_dense,
+   the cell scan, scatter and rank_gather write, except the scatter's
+   scratch (raw4, ticket, tmp). */
+bool small_binning_fits(int64_t nat, int64_t ncells);
+hipError_t launch_bin_small(const double *r, int32_t nat, const SysGeom &g, int32_t ncells,
+                            SysGeom *geo, int32_t *atom_base, int32_t *cell_base,
+                            int32_t *order_stats, const double *per_atom, const int32_t *types,
+                            int32_t *cell_first, int32_t *cell_count, double4 *spos,
+                            int4 *sinfo, int32_t *stype, int32_t *scount, int32_t *slot,
+                            hipStream_t stream);
 hipError_t launch_set_params(SysGeom *geo, const SysGeom &g, int32_t *atom_base,
                              int32_t *cell_base, int32_t nat, int32_t ncells,
                              int32_t *cell_count, int64_t nslots, int64_t *hkey,

@@ -308,33 +308,14 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const int4 *__restrict__ raw
    ((r - rel1 b1) - rel2 b2) - rel3 b3. For the shift the record keeps, per
    direction, the raw cell (periodic) or the image count
    trunc((raw - clamp(raw)) / n) (non-periodic). */
-__global__ __launch_bounds__(kBlock) void k_rank_gather(
-    const double *__restrict__ r, const int4 *__restrict__ raw4,
-    const int32_t *__restrict__ tmp, int32_t nat, const int32_t *__restrict__ cell_first,
-    const int32_t *__restrict__ cell_count, const SysGeom *__restrict__ geo,
-    const int32_t *__restrict__ atom_base, int32_t nsys,
-    const double *__restrict__ per_atom, const int32_t *__restrict__ types,
-    double4 *__restrict__ spos, int4 *__restrict__ sinfo, int32_t *__restrict__ stype,
-    int32_t *__restrict__ scount, int32_t *__restrict__ slot) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nat) return;
-    const int4 rc = raw4[a];
-    const int b = cell_first[rc.w];
-    const int m = cell_count[rc.w];
-    /* members read eight at a time: independent loads, one round trip */
-    int rank = 0;
-    for (int t0 = 0; t0 < m; t0 += 8) {
-        int v[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) v[u] = t0 + u < m ? tmp[b + t0 + u] : INT32_MAX;
-#pragma unroll
-        for (int u = 0; u < 8; u++) rank += v[u] < a ? 1 : 0;
-    }
-    const int s = b + rank;
-
-    const int sys = system_of(atom_base, nsys, a);
-    const SysGeom &g = geo[sys];
-    const int raw[3] = {rc.x, rc.y, rc.z};
+__device__ __forceinline__ void place_atom(const double *__restrict__ r, int a, int s,
+                                           const int raw[3], const SysGeom &g,
+                                           const double *__restrict__ per_atom,
+                                           const int32_t *__restrict__ types,
+                                           double4 *__restrict__ spos, int4 *__restrict__ sinfo,
+                                           int32_t *__restrict__ stype,
+                                           int32_t *__restrict__ scount,
+                                           int32_t *__restrict__ slot) {
     int rel[3], base[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -360,6 +341,34 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
     if (stype) stype[s] = types ? types[a] : 0;
     if (scount) scount[s] = 0; /* the half count accumulates here */
     if (slot) slot[a] = s;     /* atom -> cell-sorted index */
+}
+
+__global__ __launch_bounds__(kBlock) void k_rank_gather(
+    const double *__restrict__ r, const int4 *__restrict__ raw4,
+    const int32_t *__restrict__ tmp, int32_t nat, const int32_t *__restrict__ cell_first,
+    const int32_t *__restrict__ cell_count, const SysGeom *__restrict__ geo,
+    const int32_t *__restrict__ atom_base, int32_t nsys,
+    const double *__restrict__ per_atom, const int32_t *__restrict__ types,
+    double4 *__restrict__ spos, int4 *__restrict__ sinfo, int32_t *__restrict__ stype,
+    int32_t *__restrict__ scount, int32_t *__restrict__ slot) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nat) return;
+    const int4 rc = raw4[a];
+    const int b = cell_first[rc.w];
+    const int m = cell_count[rc.w];
+    /* members read eight at a time: independent loads, one round trip */
+    int rank = 0;
+    for (int t0 = 0; t0 < m; t0 += 8) {
+        int v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = t0 + u < m ? tmp[b + t0 + u] : INT32_MAX;
+#pragma unroll
+        for (int u = 0; u < 8; u++) rank += v[u] < a ? 1 : 0;
+    }
+    const int s = b + rank;
+    const int raw[3] = {rc.x, rc.y, rc.z};
+    place_atom(r, a, s, raw, geo[system_of(atom_base, nsys, a)], per_atom, types, spos, sinfo,
+               stype, scount, slot);
 }
 
 /* ---------------------------------------------------------------- scan */
@@ -444,16 +453,23 @@ __global__ __launch_bounds__(1024) void k_scan_spine(int64_t *__restrict__ parti
 /* Scan of the tiles with their offsets: with `fused`, partial holds the
    tiles' sums and each block adds up those before it (few tiles: replaces
    the spine kernel; the last block writes the totals); otherwise partial
-   holds the exclusive offsets the spine wrote. */
+   holds the exclusive offsets the spine wrote. A single tile needs no
+   partial sums at all. With gslot the input is gathered as
+   gsrc[gslot[idx]] and also stored to gout[idx] (the first phase's gather,
+   for a scan that runs without it). */
 template <typename TI, typename TO>
 __global__ __launch_bounds__(kBlock) void k_scan_apply(const TI *__restrict__ in, int64_t n,
                                                         const int64_t *__restrict__ partial,
                                                         TO *__restrict__ out, int write_total,
                                                         int fused, int64_t *__restrict__ total,
-                                                        int64_t *__restrict__ total2) {
+                                                        int64_t *__restrict__ total2,
+                                                        const int32_t *__restrict__ gsrc,
+                                                        const int32_t *__restrict__ gslot,
+                                                        int32_t *__restrict__ gout) {
     __shared__ int64_t s_tile[kScanTile];
     __shared__ int64_t s_w[kWavesPerBlock];
     __shared__ int64_t s_pre;
+    __shared__ int64_t s_last;
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
     if (fused) {
         int64_t v = 0;
@@ -471,7 +487,18 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const TI *__restrict__ in
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
         const int64_t idx = base + k * kBlock + threadIdx.x;
-        s_tile[k * kBlock + threadIdx.x] = idx < n ? static_cast<int64_t>(in[idx]) : 0;
+        int64_t v = 0;
+        if (idx < n) {
+            if (gslot) {
+                const int32_t c = gsrc[gslot[idx]];
+                gout[idx] = c;
+                v = c;
+            } else {
+                v = static_cast<int64_t>(in[idx]);
+            }
+            if (idx == n - 1) s_last = v;
+        }
+        s_tile[k * kBlock + threadIdx.x] = v;
     }
     __syncthreads();
     int64_t local[kScanItems];
@@ -497,13 +524,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const TI *__restrict__ in
     }
     if (write_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) {
         /* the last element's exclusive prefix plus itself */
-        const int64_t last = n - 1;
-        const int64_t tot = s_tile[last - base] + static_cast<int64_t>(in[last]);
+        const int64_t tot = s_tile[n - 1 - base] + s_last;
         out[n] = static_cast<TO>(tot);
     }
     if (fused && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1 && (total || total2)) {
-        const int64_t last = n - 1;
-        const int64_t tot = s_tile[last - base] + static_cast<int64_t>(in[last]);
+        const int64_t tot = s_tile[n - 1 - base] + s_last;
         if (total) *total = tot;
         if (total2) *total2 = tot;
     }
@@ -1222,6 +1247,23 @@ __global__ __launch_bounds__(kBlock) void k_fill(PairArgs A) {
     fill_cell<MODE, CUT, HASHED, FC>(A, h, part, s_lds[wib], pt_sq);
 }
 
+/* The previous binning measured, per axis, how far atoms 32 indices apart
+   lie (in cells). If one axis carries at least twice the distance of each
+   other, the input's atom order runs along it: make it the fastest of the
+   dense layout (then the next), so that cells processed one after another
+   hold atoms with nearby indices and the output rows they write lie close
+   together. */
+__device__ __forceinline__ void choose_layout(int s0, int s1, int s2, SysGeom &g) {
+    const int f = (s0 >= s1 && s0 >= s2) ? 0 : (s1 >= s2 ? 1 : 2);
+    const int u = f == 0 ? 1 : 0, v = f == 2 ? 1 : 2; /* the other two, u < v */
+    const int sf = pick3(f, s0, s1, s2), su = pick3(u, s0, s1, s2), sv = pick3(v, s0, s1, s2);
+    if (sf > 2 * su && sf > 2 * sv) {
+        g.lay[0] = f;
+        g.lay[1] = su >= sv ? u : v;
+        g.lay[2] = su >= sv ? v : u;
+    }
+}
+
 /* Device-side parameter set-up (keeps the whole build free of host-to-device
    copies, so that it can be captured in a graph): the one-system parameters,
    and the cleared grid: cell (or slot) counts to 0, hash keys (if any) to
@@ -1237,22 +1279,7 @@ __global__ void k_set_params(SysGeom *__restrict__ geo, SysGeom g, int32_t *__re
     }
     if (t != 0) return;
     if (order_stats) {
-        /* The previous binning measured, per axis, how far atoms 32 indices
-           apart lie (in cells). If one axis carries at least twice the
-           distance of each other, the input's atom order runs along it:
-           make it the fastest of the dense layout (then the next), so that
-           cells processed one after another hold atoms with nearby indices
-           and the output rows they write lie close together. */
-        const int s0 = order_stats[0], s1 = order_stats[1], s2 = order_stats[2];
-        const int f = (s0 >= s1 && s0 >= s2) ? 0 : (s1 >= s2 ? 1 : 2);
-        const int u = f == 0 ? 1 : 0, v = f == 2 ? 1 : 2; /* the other two, u < v */
-        const int sf = pick3(f, s0, s1, s2), su = pick3(u, s0, s1, s2),
-                  sv = pick3(v, s0, s1, s2);
-        if (sf > 2 * su && sf > 2 * sv) {
-            g.lay[0] = f;
-            g.lay[1] = su >= sv ? u : v;
-            g.lay[2] = su >= sv ? v : u;
-        }
+        choose_layout(order_stats[0], order_stats[1], order_stats[2], g);
         order_stats[0] = order_stats[1] = order_stats[2] = 0;
     }
     geo[0] = g;
@@ -1260,6 +1287,141 @@ __global__ void k_set_params(SysGeom *__restrict__ geo, SysGeom g, int32_t *__re
     atom_base[1] = nat;
     cell_base[0] = 0;
     cell_base[1] = ncells;
+}
+
+/*
+ * The whole binning of a small single system (dense grid, up to
+ * kSmallAtoms atoms and kSmallCells cells) in one block, replacing
+ * k_set_params, k_bin_dense, the cell scan, k_scatter and k_rank_gather
+ * with the same results: the parameters (with the layout the previous
+ * build's order sample selects) and the cell grid, cell slices in ascending
+ * atom order, the cell-sorted records. The histogram, the scan and the
+ * member lists live in LDS. The order sample for the next build is the one
+ * k_bin_dense takes at this size (atoms 0..63, pairs a, a + 32).
+ */
+constexpr int kSmallThreads = 1024;
+constexpr int kSmallAtoms = 4096;
+constexpr int kSmallCells = 4096;
+constexpr int kSmallPer = kSmallAtoms / kSmallThreads;
+
+__global__ __launch_bounds__(kSmallThreads) void k_bin_small(
+    const double *__restrict__ r, int32_t nat, SysGeom g0, int32_t ncells,
+    SysGeom *__restrict__ geo, int32_t *__restrict__ atom_base, int32_t *__restrict__ cell_base,
+    int32_t *__restrict__ order_stats, const double *__restrict__ per_atom,
+    const int32_t *__restrict__ types, int32_t *__restrict__ cell_first,
+    int32_t *__restrict__ cell_count, double4 *__restrict__ spos, int4 *__restrict__ sinfo,
+    int32_t *__restrict__ stype, int32_t *__restrict__ scount, int32_t *__restrict__ slot) {
+    __shared__ int s_cnt[kSmallCells];
+    __shared__ int s_first[kSmallCells];
+    __shared__ int s_tmp[kSmallAtoms];
+    __shared__ int s_w[kSmallThreads / kWave];
+    __shared__ SysGeom s_g;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        SysGeom h = g0;
+        if (order_stats) choose_layout(order_stats[0], order_stats[1], order_stats[2], h);
+        s_g = h;
+        geo[0] = h;
+        atom_base[0] = 0;
+        atom_base[1] = nat;
+        cell_base[0] = 0;
+        cell_base[1] = ncells;
+    }
+    for (int c = tid; c < ncells; c += kSmallThreads) s_cnt[c] = 0;
+    __syncthreads();
+    const SysGeom &g = s_g;
+
+    /* cells, and tickets in arrival order */
+    int raw[kSmallPer][3], cell[kSmallPer], tk[kSmallPer];
+    int c0[3] = {0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < kSmallPer; u++) {
+        const int a = tid + u * kSmallThreads;
+        cell[u] = -1;
+        tk[u] = 0;
+        raw[u][0] = raw[u][1] = raw[u][2] = 0;
+        if (a < nat) {
+            raw_cell_of(r, a, g, raw[u]);
+            int c[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                c[k] = g.pbc[k] ? wrap_idx(raw[u][k], g.n[k]) : clamp_idx(raw[u][k], g.n[k]);
+            cell[u] = dense_index(g, c[0], c[1], c[2]);
+            tk[u] = atomicAdd(&s_cnt[cell[u]], 1);
+            if (u == 0) {
+                c0[0] = c[0];
+                c0[1] = c[1];
+                c0[2] = c[2];
+            }
+        }
+    }
+    if (order_stats && tid < kWave) {
+        /* all lanes take part in the shuffles and sums; the store waits for
+           every thread's read of the previous sample (barriers below) */
+        const bool pair = tid < 32 && tid + 32 < nat;
+        int moved[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int cn = __shfl_down(c0[k], 32, kWave);
+            int dd = cn - c0[k];
+            if (2 * dd > g.n[k]) dd -= g.n[k];
+            if (2 * dd < -g.n[k]) dd += g.n[k];
+            moved[k] = wave_sum(pair ? min(dd < 0 ? -dd : dd, 64) : 0);
+        }
+        if (tid == 0) {
+            s_w[0] = moved[0];
+            s_w[1] = moved[1];
+            s_w[2] = moved[2];
+        }
+    }
+    __syncthreads();
+    if (order_stats && tid == 0) {
+        order_stats[0] = s_w[0];
+        order_stats[1] = s_w[1];
+        order_stats[2] = s_w[2];
+    }
+    __syncthreads();
+
+    /* exclusive scan of the cell counts: kSmallPer consecutive cells per
+       thread, then the thread sums across the block */
+    int loc[kSmallPer];
+    int sum = 0;
+#pragma unroll
+    for (int u = 0; u < kSmallPer; u++) {
+        const int c = tid * kSmallPer + u;
+        loc[u] = sum;
+        sum += c < ncells ? s_cnt[c] : 0;
+    }
+    const int incl = wave_incl_scan(sum);
+    if (lane_id() == kWave - 1) s_w[tid / kWave] = incl;
+    __syncthreads();
+    int pre = incl - sum;
+    for (int w = 0; w < tid / kWave; w++) pre += s_w[w];
+#pragma unroll
+    for (int u = 0; u < kSmallPer; u++) {
+        const int c = tid * kSmallPer + u;
+        if (c < ncells) {
+            s_first[c] = pre + loc[u];
+            cell_first[c] = pre + loc[u];
+            cell_count[c] = s_cnt[c];
+        }
+    }
+    __syncthreads();
+
+    /* member lists, then the stable rank: members with a smaller index */
+#pragma unroll
+    for (int u = 0; u < kSmallPer; u++)
+        if (cell[u] >= 0) s_tmp[s_first[cell[u]] + tk[u]] = tid + u * kSmallThreads;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kSmallPer; u++) {
+        if (cell[u] < 0) continue;
+        const int a = tid + u * kSmallThreads;
+        const int b = s_first[cell[u]], m = s_cnt[cell[u]];
+        int rank = 0;
+        for (int t = 0; t < m; t++) rank += s_tmp[b + t] < a ? 1 : 0;
+        place_atom(r, a, b + rank, raw[u], g, per_atom, types, spos, sinfo, stype, scount, slot);
+    }
 }
 
 /* ---------------------------------------------------------------- backward */
@@ -1532,9 +1694,19 @@ static hipError_t scan_impl(const TI *in, int64_t n, TO *out, int64_t *partial,
         return hipGetLastError();
     }
     const int64_t nb = (n + kScanTile - 1) / kScanTile;
+    int32_t *gout = reinterpret_cast<int32_t *>(const_cast<TI *>(in));
+    if (nb == 1) {
+        /* one tile: the apply alone (its block has no tiles before it), with
+           the gather folded in */
+        k_scan_apply<TI, TO><<<1, kBlock, 0, stream>>>(in, n, partial, out,
+                                                       write_total ? 1 : 0, 1, total, total2,
+                                                       gslot ? gsrc : nullptr, gslot,
+                                                       gslot ? gout : nullptr);
+        return hipGetLastError();
+    }
     if (gslot)
-        k_scan_reduce_gather<<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(
-            gsrc, gslot, reinterpret_cast<int32_t *>(const_cast<TI *>(in)), n, partial);
+        k_scan_reduce_gather<<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(gsrc, gslot, gout,
+                                                                               n, partial);
     else
         k_scan_reduce<TI><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(in, n, partial);
     /* up to this many tiles, each apply block adds up the tile sums before
@@ -1544,7 +1716,7 @@ static hipError_t scan_impl(const TI *in, int64_t n, TO *out, int64_t *partial,
     if (!fused) k_scan_spine<<<1, 1024, 0, stream>>>(partial, nb, total, total2);
     k_scan_apply<TI, TO><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(
         in, n, partial, out, write_total ? 1 : 0, fused ? 1 : 0, fused ? total : nullptr,
-        fused ? total2 : nullptr);
+        fused ? total2 : nullptr, nullptr, nullptr, nullptr);
     return hipGetLastError();
 }
 
@@ -1636,6 +1808,23 @@ hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
         launch_pairs_h<CUT_PER_TYPE>(mode, hashed, a, nb, stream);
     else
         launch_pairs_h<CUT_GLOBAL>(mode, hashed, a, nb, stream);
+    return hipGetLastError();
+}
+
+bool small_binning_fits(int64_t nat, int64_t ncells) {
+    return nat > 0 && nat <= kSmallAtoms && ncells > 0 && ncells <= kSmallCells;
+}
+
+hipError_t launch_bin_small(const double *r, int32_t nat, const SysGeom &g, int32_t ncells,
+                            SysGeom *geo, int32_t *atom_base, int32_t *cell_base,
+                            int32_t *order_stats, const double *per_atom, const int32_t *types,
+                            int32_t *cell_first, int32_t *cell_count, double4 *spos,
+                            int4 *sinfo, int32_t *stype, int32_t *scount, int32_t *slot,
+                            hipStream_t stream) {
+    if (!small_binning_fits(nat, ncells)) return hipErrorInvalidValue;
+    k_bin_small<<<1, kSmallThreads, 0, stream>>>(r, nat, g, ncells, geo, atom_base, cell_base,
+                                                 order_stats, per_atom, types, cell_first,
+                                                 cell_count, spos, sinfo, stype, scount, slot);
     return hipGetLastError();
 }
 

@@ -346,3 +346,59 @@ def test_build_sequence_fuzz(gpu, seed):
     for rep in range(2):
         _check(pos, cell, pbc, cutoff, numbers=numbers, backends=(0, 1),
                msg=f"seed={seed} build {rep}")
+
+
+@pytest.mark.parametrize("case", ["atoms", "cells"])
+def test_small_binning_limits(gpu, case):
+    """Single dense systems of up to 4096 atoms and 4096 cells are binned by
+    one block (grid, scan, member lists and stable ranks in LDS); larger
+    ones by the multi-kernel binning. Sizes on both sides of each limit,
+    built one after another on the same plan (the paths hand the plan's
+    parameters and order sample to each other), bit for bit against the
+    oracle."""
+    rng = np.random.default_rng(71 if case == "atoms" else 72)
+    if case == "atoms":
+        # 12 cells a side (1728 cells), 4095 / 4096 / 4097 atoms
+        L, rc = 12.5, 1.0
+        sizes = [(4096, (L, L, L)), (4097, (L, L, L)), (4095, (L, L, L)), (4096, (L, L, L))]
+    else:
+        # 16 x 16 x 16 = 4096 cells against 17 x 16 x 16 = 4352
+        rc = 1.0
+        sizes = [(2000, (16.5, 16.5, 16.5)), (2000, (17.5, 16.5, 16.5)),
+                 (2000, (16.5, 16.5, 16.5))]
+    for k, (n, edges) in enumerate(sizes):
+        cell = np.diag(edges)
+        pos = rng.uniform(0.0, 1.0, (n, 3)) @ cell
+        _check(pos, cell, True, rc, backends=(-1, 0), msg=f"{case} build {k}: {n} atoms")
+
+
+@pytest.mark.parametrize("order", ["zyx", "xyz", "random"])
+def test_small_binning_layout(gpu, order):
+    """The one-block binning takes the layout the previous build's order
+    sample selects and samples the input for the next build: a 6x7x8 fcc
+    lattice (1344 atoms) in C order, Fortran order and random order, built
+    three times in a row, bit for bit against the oracle each time."""
+    a, reps = 3.6, (6, 7, 8)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    grids = np.stack(np.meshgrid(*[np.arange(r) for r in reps], indexing="ij"), -1)
+    if order == "xyz":
+        grids = grids.transpose(2, 1, 0, 3)
+    pos = ((grids.reshape(-1, 1, 3) + basis[None]) * a).reshape(-1, 3)
+    rng = np.random.default_rng(19)
+    if order == "random":
+        pos = pos[rng.permutation(len(pos))]
+    pos = pos + rng.normal(0.0, 0.05, pos.shape)
+    cell = np.diag(np.asarray(reps, float) * a)
+    for rep in range(3):
+        _check(pos, cell, True, 4.2, msg=f"{order} build {rep}")
+
+
+@pytest.mark.parametrize("pbc", [True, False])
+def test_small_binning_one_full_cell(gpu, pbc):
+    """600 atoms in a 4 A box at rc 3: a single cell holds every atom (the
+    stable rank compares against 600 members), and with pbc its 27 images
+    are the stencil."""
+    rng = np.random.default_rng(73)
+    cell = 4.0 * np.eye(3)
+    pos = rng.uniform(0.0, 4.0, (600, 3))
+    _check(pos, cell, pbc, 3.0, backends=(-1, 0, 1))
