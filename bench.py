@@ -449,8 +449,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
-              "(the neighbour-list engine runs on a single device; the multi-GPU "
-              "sharding is a design in DESIGN.md, not a measured path)", file=sys.stderr)
+              "(the neighbour-list engine runs on a single device; splitting a build "
+              "over several GPUs by atom range is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import ctypes
