@@ -386,8 +386,77 @@ def leg_autograd(lib, _lib, torch, dev, steps, warmup, with_cpu):
                "dL/dd (108 B/pair) and writes dL/dr (24 B/atom) and dL/dcell"}
 
 
+def leg_lj(lib, _lib, torch, dev, steps, warmup, with_cpu):
+    """The fused Lennard-Jones consumer on the fcc 1M workload (eps 0.4,
+    sigma 2.3, rc 5 A): forces, energy and virial straight from the cell
+    grid, against building the ijdD list (count + fill of i, D, d) and
+    summing the same forces over it with torch."""
+    import ctypes
+
+    import numpy as np
+
+    from matscipy_neighbours_amd import workloads
+    stream = torch.cuda.current_stream(dev)
+    w = workloads.fcc_bulk()
+    N = len(w["positions"])
+    cell = np.ascontiguousarray(w["cell"], dtype=np.float64)
+    inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+    pbc = np.ones(3, np.int32)
+    co = np.zeros(3)
+    pos = torch.from_numpy(w["positions"]).to(dev)
+    eps, sig, rc = 0.4, 2.3, 5.0
+    plan = _lib.Plan(0)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    P_ = _lib.ptr
+    forces = torch.empty((N, 3), dtype=torch.float64, device=dev)
+    energy = torch.empty(1, dtype=torch.float64, device=dev)
+    virial = torch.empty((3, 3), dtype=torch.float64, device=dev)
+
+    def fused():
+        _lib.check(lib.mnl_plan_lj(plan.handle, sp, P_(co), P_(cell), P_(inv), P_(pbc), N,
+                                   P_(pos), rc, eps, sig, P_(forces), P_(energy), P_(virial)))
+    ms_fused = _time_steps(torch, stream, fused, steps, warmup)
+
+    step, P = _plan_step_fn(lib, _lib, torch, dev, w, flags="iDd")
+    o = step.out
+    f_list = torch.empty_like(forces)
+    counts = torch.empty(N, dtype=torch.int32, device=dev)
+    # the rows are i-sorted: a segmented sum over the plan's per-atom counts
+    # (torch.segment_reduce), or index_add_ where that is not available
+    try:
+        torch.segment_reduce(torch.ones((4, 3), dtype=torch.float64, device=dev), "sum",
+                             lengths=torch.tensor([1, 3], device=dev))
+        consumer = "segment_reduce"
+    except (RuntimeError, NotImplementedError):
+        consumer = "index_add_"
+
+    def via_list():
+        step()
+        r2 = o["d"] * o["d"]
+        s6 = (sig * sig / r2) ** 3
+        coef = -24.0 * eps * (2.0 * s6 * s6 - s6) / r2
+        terms = coef[:, None] * o["D"]
+        if consumer == "segment_reduce":
+            _lib.check(lib.mnl_plan_counts(step.plan.handle, step.sp, P_(counts)))
+            f_list.copy_(torch.segment_reduce(terms, "sum", lengths=counts))
+        else:
+            f_list.zero_().index_add_(0, o["i"], terms)
+    ms_list = _time_steps(torch, stream, via_list, steps, warmup)
+    fused()
+    torch.cuda.synchronize()
+    scale = float(forces.abs().max())
+    return "fcc_1M_lj_fused", {
+        "atoms": N, "pairs": P, "fused_ms": ms_fused, "via_list_ms": ms_list,
+        "speedup": ms_list / ms_fused,
+        "max_force_difference_rel": float((forces - f_list).abs().max()) / scale,
+        "how": "fused: mnl_plan_lj (binning + one traversal accumulating forces, energy and "
+               "virial, no pair list written); via list: count + fill of i, D, d "
+               f"into pre-allocated outputs, then the forces summed with torch {consumer} "
+               "over the i-sorted rows; HIP events over the steps"}
+
+
 OTHER_LEGS = {"latency": leg_latency, "slab": leg_slab, "fcc_f32": leg_fcc_f32,
-              "autograd": leg_autograd}
+              "autograd": leg_autograd, "lj": leg_lj}
 
 
 def other_workloads(lib, _lib, torch, dev, steps, warmup, with_cpu):

@@ -329,6 +329,31 @@ int mnl_plan_pair_backward(mnl_plan *plan, void *stream, int64_t npairs, int32_t
                            const void *grad_absdist, int float32,
                            double *grad_positions, double *grad_cell);
 
+/* ------------------------------------------------------- fused consumer */
+
+/*
+ * Lennard-Jones forces, energy and virial of one system, computed straight
+ * from the cell grid without writing a pair list. The reference has no
+ * such entry point: its examples/lj_langevin/lj_langevin_cpu.cc:62-98
+ * computes the same forces from an "ij" list (eps = sigma = 1, open
+ * boundaries), and the build is for callers that want forces rather than
+ * the list. The pairs are the neighbour list's with a uniform cutoff:
+ * r < cutoff, D = r_j - r_i + S.cell, images included. With
+ * phi(r) = 4 eps ((sigma/r)^12 - (sigma/r)^6), unshifted:
+ *   forces [nat][3] float64: f_i = sum_j coef D_ij,
+ *     coef = -24 eps (2 (sigma/r)^12 - (sigma/r)^6) / r^2;
+ *   energy [1] float64 (may be NULL): sum over pairs of phi;
+ *   virial [3][3] float64 (may be NULL): -1/2 sum over ordered pairs of
+ *     coef D (x) D, i.e. sum over pairs of r_ij (x) f_ij.
+ * All pointers [device]; asynchronous. Forces are float64 sums with
+ * atomics (order-dependent in the last bits); energy and virial are
+ * reduced in a fixed order. The plan's count state is cleared.
+ */
+int mnl_plan_lj(mnl_plan *plan, void *stream, const double origin[3], const double cell[9],
+                const double inv_cell[9], const int32_t pbc[3], int32_t nat,
+                const double *positions, double cutoff, double epsilon, double sigma,
+                double *forces, double *energy, double *virial);
+
 /* ---------------------------------------------------------------- lists
  * Post-processing of i-sorted pair lists, all on the device.
  */

@@ -123,7 +123,7 @@ struct mnl_plan {
     DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, scount, slot,
         counts, order_stats,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
-        htypes, cell_partial, tri_m, tri_cnt, tri_off, tri_vidx, tri_flag;
+        htypes, cell_partial, tri_m, tri_cnt, tri_off, tri_vidx, tri_flag, lj;
     int64_t *host_total = nullptr; /* pinned */
     /* state of the last count */
     bool counted = false;
@@ -672,7 +672,7 @@ void mnl_plan_destroy(mnl_plan *p) {
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
                       &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
                       &p->cell_partial, &p->tri_m,   &p->tri_cnt,  &p->tri_off,
-                      &p->tri_vidx,  &p->tri_flag};
+                      &p->tri_vidx,  &p->tri_flag,  &p->lj};
     for (DevBuf *b : bufs) b->release();
     if (p->host_total) (void)hipHostFree(p->host_total);
     delete p;
@@ -1141,6 +1141,36 @@ int mnl_plan_pair_backward(mnl_plan *p, void *stream, int64_t npairs, int32_t na
     MNL_HIP(launch_pair_backward(npairs, nat, first, secnd, shift, distvec, absdist,
                                  grad_distvec, grad_absdist, float32 != 0, grad_positions,
                                  grad_cell, p->cell_partial.as<double>(), st));
+    return MNL_SUCCESS;
+}
+
+int mnl_plan_lj(mnl_plan *p, void *stream, const double origin[3], const double cell[9],
+                const double inv_cell[9], const int32_t pbc[3], int32_t nat,
+                const double *positions, double cutoff, double epsilon, double sigma,
+                double *forces, double *energy, double *virial) {
+    if (!p) return fail("plan is NULL");
+    if (nat < 0) return fail("negative atom count");
+    if (nat > 0 && (!positions || !forces)) return fail("positions and forces are required");
+    if (!(sigma == sigma) || !(epsilon == epsilon)) return fail("epsilon or sigma is NaN");
+    DeviceScope ds(p->device);
+    if (!ds.ok) return fail("cannot switch to the plan's device");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    /* the cell grid only (no count): the traversal below is the consumer */
+    if (enqueue_count(p, st, origin, cell, inv_cell, pbc, nat, positions, cutoff, nullptr,
+                      nullptr, 0, nullptr, -1, false) != MNL_SUCCESS)
+        return MNL_ERROR;
+    const PairArgs a = pair_args(p);
+    const size_t need = sizeof(double) * static_cast<size_t>(lj_scratch_doubles(a));
+    if (need > p->lj.cap) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        MNL_HIP(hipStreamIsCapturing(st, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return fail("mnl workspace must grow during graph capture");
+        MNL_HIP(hipStreamSynchronize(st));
+        MNL_HIP(p->lj.ensure(need));
+    }
+    MNL_HIP(launch_lj(a, p->backend == MNL_BACKEND_HASHED, epsilon, sigma,
+                      p->slot.as<int32_t>(), p->lj.as<double>(), forces, energy, virial, st));
     return MNL_SUCCESS;
 }
 

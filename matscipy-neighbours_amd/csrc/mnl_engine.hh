@@ -169,6 +169,16 @@ hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
    cell_count[0..nslots) = 0 and, if hkey is given, hkey[0..nslots) = -1.
    order_stats (may be NULL): the previous binning's per-axis counts, which
    choose the dense layout, then cleared. */
+/* Lennard-Jones forces [nat][3], energy and virial [3][3] (energy and
+   virial may be NULL) from the binned grid of the plan's last binning,
+   without a pair list; global cutoff only. scratch holds
+   lj_scratch_doubles(args) doubles; slot maps atoms to cell-sorted
+   indices. */
+int64_t lj_scratch_doubles(const PairArgs &a);
+hipError_t launch_lj(const PairArgs &args, bool hashed, double epsilon, double sigma,
+                     const int32_t *slot, double *scratch, double *forces, double *energy,
+                     double *virial, hipStream_t stream);
+
 /* The whole binning of one small dense system (at most 4096 atoms and 4096
    cells: small_binning_fits) in a single block: writes the parameters (with
    the layout chosen from order_stats) and everything set_params, bin_dense,

@@ -1040,6 +1040,139 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         count_cell<CUT, HASHED>(A, h, part, s_lds[wib], pt_sq);
 }
 
+/* ------------------------------------------------------- fused consumer */
+
+/*
+ * Lennard-Jones forces, energy and virial straight from the cell grid, with
+ * no pair list. The traversal is the full count's: every ordered pair with
+ * i a home atom and j a candidate, the pair's own test (r2 < rc^2, i != j
+ * at zero offset) and its D = r_j - r_i. With s6 = (sigma^2 / r2)^3:
+ *   coef = -24 eps (2 s6^2 - s6) / r2, force on i coef D, on j -coef D
+ * (examples/lj_langevin/lj_langevin_cpu.cc:76-90, there with eps = sigma =
+ * 1 and D recomputed from the positions). Each lane sums the force on its
+ * candidate from the home atoms and adds it to the cell-sorted force arrays
+ * (one atomic per lane, component and chunk): over all home cells every
+ * atom receives the force of each of its neighbours once, and no sum runs
+ * across lanes. Per ordered pair the lane also adds half the pair energy
+ * 4 eps (s6^2 - s6) and half the virial -coef D (x) D; these are summed
+ * per block into partial[block][10] (energy, then the 3x3 virial row by
+ * row) and reduced in a fixed order.
+ */
+template <bool HASHED>
+__device__ __forceinline__ void lj_cell(const PairArgs &A, int h, int part, SegLds &S,
+                                        double eps4, double sig2, double *__restrict__ fx,
+                                        double *__restrict__ fy, double *__restrict__ fz,
+                                        double acc[7]) {
+    HomeCell H;
+    if (!load_home<HASHED>(A, h, H)) return;
+    const int lane = lane_id();
+    int lo, hi;
+    home_part(H.m, part, A.split, lo, hi);
+    for (int grp0 = lo; grp0 < hi; grp0 += kWave) {
+        const int mg = uniform(min(kWave, hi - grp0));
+        for (int sg = 0; sg < H.nseg; sg += kWave) {
+            const int nsg = min(kWave, H.nseg - sg);
+            int self_start;
+            const int total = build_segments<HASHED>(A, H, sg, nsg, lane, S, &self_start);
+            const int self_base = self_start >= 0 ? self_start + grp0 : -(1 << 30);
+            for (int cb = 0; cb < total; cb += kWave) {
+                Cand C;
+                load_candidate<CUT_GLOBAL, false>(A, H, S, nsg, cb + lane, total, C);
+                const int c = cb + lane;
+                double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+                for (int t = 0; t < mg; t++) {
+                    const int si = H.hb + grp0 + t;
+                    const double4 pi = load_const(A.spos, si);
+                    double d0, d1, d2;
+                    const bool ok = pair_test<CUT_GLOBAL>(A, H, C, si, pi, d0, d1, d2, nullptr) &
+                                    (c != self_base + t);
+                    if (ok) {
+                        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                        const double ir2 = 1.0 / r2;
+                        const double s2 = sig2 * ir2;
+                        const double s6 = s2 * s2 * s2;
+                        const double coef = -6.0 * eps4 * ir2 * (2.0 * s6 * s6 - s6);
+                        const double g0 = coef * d0, g1 = coef * d1, g2 = coef * d2;
+                        f0 -= g0;
+                        f1 -= g1;
+                        f2 -= g2;
+                        acc[0] += 0.5 * eps4 * (s6 * s6 - s6);
+                        acc[1] -= 0.5 * g0 * d0;
+                        acc[2] -= 0.5 * g1 * d1;
+                        acc[3] -= 0.5 * g2 * d2;
+                        acc[4] -= 0.5 * g0 * d1;
+                        acc[5] -= 0.5 * g0 * d2;
+                        acc[6] -= 0.5 * g1 * d2;
+                    }
+                }
+                if (c < total && (f0 != 0.0 || f1 != 0.0 || f2 != 0.0)) {
+                    unsafeAtomicAdd(fx + C.j, f0);
+                    unsafeAtomicAdd(fy + C.j, f1);
+                    unsafeAtomicAdd(fz + C.j, f2);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
+template <bool HASHED>
+__global__ __launch_bounds__(kBlock) void k_lj(PairArgs A, double eps4, double sig2,
+                                               double *__restrict__ fx, double *__restrict__ fy,
+                                               double *__restrict__ fz,
+                                               double *__restrict__ partial) {
+    __shared__ SegLds s_lds[kWavesPerBlock];
+    __shared__ double s_red[kWavesPerBlock][7];
+    const int wib = uniform(threadIdx.x / kWave);
+    const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
+    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int h, part;
+    wave_home(A, blk * kWavesPerBlock + wib, h, part);
+    if (h < A.home_cells) lj_cell<HASHED>(A, h, part, s_lds[wib], eps4, sig2, fx, fy, fz, acc);
+#pragma unroll
+    for (int k = 0; k < 7; k++) acc[k] = wave_sum(acc[k]);
+    if (lane_id() == 0) {
+#pragma unroll
+        for (int k = 0; k < 7; k++) s_red[wib][k] = acc[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        double t = 0.0;
+        for (int q = 0; q < kWavesPerBlock; q++) t += s_red[q][threadIdx.x];
+        s_red[0][threadIdx.x] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 10) {
+        /* energy, then the symmetric virial row by row */
+        constexpr int kFrom[10] = {0, 1, 4, 5, 4, 2, 6, 5, 6, 3};
+        partial[static_cast<int64_t>(blockIdx.x) * 10 + threadIdx.x] = s_red[0][kFrom[threadIdx.x]];
+    }
+}
+
+/* Forces to atom order ([nat][3]: atom a's sums sit at its cell-sorted
+   index), and the reduced energy and virial to the caller's pointers. */
+__global__ __launch_bounds__(kBlock) void k_lj_finish(const double *__restrict__ fx,
+                                                      const double *__restrict__ fy,
+                                                      const double *__restrict__ fz,
+                                                      const int32_t *__restrict__ slot,
+                                                      int32_t nat, const double *__restrict__ sums,
+                                                      double *__restrict__ forces,
+                                                      double *__restrict__ energy,
+                                                      double *__restrict__ virial) {
+    const int64_t a = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (a < nat) {
+        const int s = slot[a];
+        forces[3 * a + 0] = fx[s];
+        forces[3 * a + 1] = fy[s];
+        forces[3 * a + 2] = fz[s];
+    }
+    if (a == 0) {
+        if (energy) energy[0] = sums[0];
+        if (virial)
+            for (int k = 0; k < 9; k++) virial[k] = sums[1 + k];
+    }
+}
+
 /* ------------------------------------------------------------- fill pass */
 
 /* Candidate chunks held in registers while the home atoms are swept (the
@@ -1581,21 +1714,27 @@ __global__ __launch_bounds__(kBlock) void k_soa_to_aos(const double *__restrict_
     out[t] = soa[k * nat + a];
 }
 
-/* Sum of nb per-block cell partials in a fixed order (one block): each
-   thread adds a strided subset, then a fixed-shape tree over the block. */
+/* Sums of per-block partials (rows of ncomp components) in a fixed order:
+   block g adds rows [g * per, min(nb, (g + 1) * per)) into row g of
+   grad_cell, each thread a strided subset, then a fixed-shape tree over
+   the block. One block with per >= nb sums all rows. */
 __global__ __launch_bounds__(kBlock) void k_cell_reduce(const double *__restrict__ partial,
-                                                        int nb, double *__restrict__ grad_cell) {
+                                                        int nb, double *__restrict__ grad_cell,
+                                                        int ncomp = 9, int per = INT32_MAX) {
     __shared__ double s_sum[kBlock];
-    for (int k = 0; k < 9; k++) {
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * per;
+    const int64_t r1 = min(static_cast<int64_t>(nb), r0 + per);
+    for (int k = 0; k < ncomp; k++) {
         double t = 0.0;
-        for (int b = threadIdx.x; b < nb; b += kBlock) t += partial[static_cast<int64_t>(b) * 9 + k];
+        for (int64_t b = r0 + threadIdx.x; b < r1; b += kBlock)
+            t += partial[b * ncomp + k];
         s_sum[threadIdx.x] = t;
         __syncthreads();
         for (int w = kBlock / 2; w > 0; w >>= 1) {
             if (static_cast<int>(threadIdx.x) < w) s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
             __syncthreads();
         }
-        if (threadIdx.x == 0) grad_cell[k] = s_sum[0];
+        if (threadIdx.x == 0) grad_cell[static_cast<int64_t>(blockIdx.x) * ncomp + k] = s_sum[0];
         __syncthreads();
     }
 }
@@ -1885,6 +2024,51 @@ hipError_t launch_pair_backward(int64_t npairs, int32_t nat, const int32_t *firs
         k_fill_f64<<<1, kWave, 0, stream>>>(grad_cell, 9, 0.0);
     }
     if (nat > 0) k_soa_to_aos<<<grid_for(n3), kBlock, 0, stream>>>(soa, nat, grad_pos);
+    return hipGetLastError();
+}
+
+/* First level of the energy/virial reduction: this many blocks, each over a
+   contiguous range of the per-block partials. */
+constexpr int kLjReduceBlocks = 64;
+
+int64_t lj_scratch_doubles(const PairArgs &a) {
+    const int64_t waves = static_cast<int64_t>(a.home_cells) * std::max(a.split, 1);
+    const int64_t nb = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    return 3 * static_cast<int64_t>(std::max(a.nat, 1)) + 10 * std::max<int64_t>(nb, 1) +
+           10 * kLjReduceBlocks + 10;
+}
+
+hipError_t launch_lj(const PairArgs &args, bool hashed, double epsilon, double sigma,
+                     const int32_t *slot, double *scratch, double *forces, double *energy,
+                     double *virial, hipStream_t stream) {
+    const int32_t nat = args.nat;
+    const int64_t waves = static_cast<int64_t>(args.home_cells) * std::max(args.split, 1);
+    const int64_t nb = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    double *fx = scratch, *fy = scratch + nat, *fz = scratch + 2 * int64_t(nat);
+    double *partial = scratch + 3 * int64_t(nat);
+    double *partial2 = partial + 10 * std::max<int64_t>(nb, 1);
+    double *sums = partial2 + 10 * kLjReduceBlocks;
+    if (nat > 0) k_fill_f64<<<grid_for(3 * int64_t(nat)), kBlock, 0, stream>>>(scratch,
+                                                                              3 * int64_t(nat), 0.0);
+    if (nb > 0) {
+        PairArgs a = args;
+        a.xcd_blocks = static_cast<int32_t>(nb);
+        const double eps4 = 4.0 * epsilon, sig2 = sigma * sigma;
+        if (hashed)
+            k_lj<true><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(a, eps4, sig2, fx, fy, fz,
+                                                                         partial);
+        else
+            k_lj<false><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(a, eps4, sig2, fx, fy, fz,
+                                                                          partial);
+        const int per = static_cast<int>((nb + kLjReduceBlocks - 1) / kLjReduceBlocks);
+        k_cell_reduce<<<kLjReduceBlocks, kBlock, 0, stream>>>(partial, static_cast<int>(nb),
+                                                              partial2, 10, per);
+        k_cell_reduce<<<1, kBlock, 0, stream>>>(partial2, kLjReduceBlocks, sums, 10);
+    } else {
+        k_fill_f64<<<1, kWave, 0, stream>>>(sums, 10, 0.0);
+    }
+    k_lj_finish<<<grid_for(std::max<int64_t>(nat, 1)), kBlock, 0, stream>>>(
+        fx, fy, fz, slot, nat, sums, forces, energy, virial);
     return hipGetLastError();
 }
 

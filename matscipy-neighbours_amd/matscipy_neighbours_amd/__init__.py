@@ -12,6 +12,7 @@ from .neighbours import (
     coordination,
     first_neighbours,
     get_jump_indicies,
+    lennard_jones,
     mic,
     neighbour_list,
     neighbour_list_batch,
@@ -29,5 +30,6 @@ __all__ = [
     "triplet_list",
     "mic",
     "coordination",
+    "lennard_jones",
     "DLPackTensor",
 ]

@@ -537,6 +537,46 @@ def coordination(atoms=None, cutoff=None, *, positions=None, cell=None, pbc=None
     return _export([counts], want_dev, array_namespace)[0]
 
 
+def lennard_jones(atoms=None, cutoff=None, *, epsilon=1.0, sigma=1.0, positions=None,
+                  cell=None, pbc=None, cell_origin=None, device=None, array_namespace=None):
+    """Lennard-Jones ``(energy, forces, virial)`` over the neighbour list's
+    pairs (r < cutoff, periodic images included), computed on the GPU
+    straight from the cell grid: the pair list is never written.
+
+    phi(r) = 4 epsilon ((sigma/r)^12 - (sigma/r)^6), unshifted; forces (n, 3)
+    f_i = sum_j coef D_ij with coef = -24 epsilon (2 (sigma/r)^12 -
+    (sigma/r)^6) / r^2 (the force loop of the reference's
+    examples/lj_langevin/lj_langevin_cpu.cc:76-90, there with epsilon =
+    sigma = 1); virial (3, 3) = sum over pairs of r_ij (x) f_ij. Host input
+    gives a float and numpy arrays, device input device tensors."""
+    if cutoff is None:
+        raise ValueError("Please provide a value for the cutoff radius.")
+    if not isinstance(cutoff, (float, int)) or isinstance(cutoff, bool):
+        raise TypeError("lennard_jones takes a single float cutoff.")
+    pos_t, co, ce, inv, pb, nums, on_device, want_dev = _prepare(
+        atoms, positions, cell, pbc, None, cell_origin, device)
+    b = _Build(pos_t, co, ce, inv, pb, float(cutoff), nums)
+    torch = _torch()
+    lib = _lib.load()
+    n = b.nat
+    idx = b.dev.index if b.dev.index is not None else torch.cuda.current_device()
+    plan = _lib.plan_for(idx)
+    kw = dict(dtype=torch.float64, device=b.dev)
+    forces = torch.empty((n, 3), **kw)
+    energy = torch.empty(1, **kw)
+    virial = torch.empty((3, 3), **kw)
+    with plan.lock, torch.cuda.device(idx):
+        sp = _lib.ctypes.c_void_p(torch.cuda.current_stream(idx).cuda_stream)
+        _lib.check(lib.mnl_plan_lj(
+            plan.handle, sp, _lib.ptr(b.co), _lib.ptr(b.ce), _lib.ptr(b.inv), _lib.ptr(b.pbc),
+            n, _lib.ptr(b.pos), b.rc, float(epsilon), float(sigma), _lib.ptr(forces),
+            _lib.ptr(energy), _lib.ptr(virial)))
+    if not want_dev and array_namespace is None:
+        return float(energy.item()), forces.cpu().numpy(), virial.cpu().numpy()
+    e, f, w = _export([energy.reshape(()), forces, virial], want_dev, array_namespace)
+    return e, f, w
+
+
 def _int32_on_device(x):
     """(tensor on the GPU, was_device) for an int32 index array."""
     torch = _torch()
