@@ -61,3 +61,20 @@ def test_no_gpu_fails_loudly():
     with pytest.raises(RuntimeError, match="no HIP device"):
         neighbour_list("ij", positions=np.zeros((2, 3)), cell=np.eye(3) * 5, pbc=True,
                        cutoff=1.0)
+
+
+def test_argument_errors_return_status_without_device():
+    """Entry points check their arguments before any device work: a NULL plan
+    or a missing required buffer gives status -1 and a message, on any host."""
+    from matscipy_neighbours_amd import _lib
+    lib = _lib.load()
+    z3 = np.zeros(3)
+    eye = np.eye(3)
+    pbc = np.ones(3, np.int32)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    assert lib.mnl_plan_lj(None, None, P(z3), P(eye), P(eye), P(pbc), 2, None, 1.0, 1.0, 1.0,
+                           None, None, None) == -1
+    assert b"plan is NULL" in lib.mnl_last_error()
+    assert lib.mnl_plan_pair_backward(None, None, 0, 0, None, None, None, None, None, None,
+                                      None, 0, None, None) == -1
+    assert b"plan is NULL" in lib.mnl_last_error()

@@ -34,3 +34,13 @@ def test_argument_errors_before_device_work():
         neighbour_list("ij", positions=np.zeros((2, 3)), cell=np.eye(3))
     with pytest.raises(ValueError, match="Unsupported quantity"):
         neighbour_list("ijx", positions=np.zeros((2, 3)), cell=np.eye(3), cutoff=1.0)
+
+
+def test_lennard_jones_argument_errors():
+    from matscipy_neighbours_amd import lennard_jones
+    with pytest.raises(ValueError, match="cutoff"):
+        lennard_jones(positions=np.zeros((2, 3)), cell=np.eye(3))
+    with pytest.raises(TypeError, match="single float cutoff"):
+        lennard_jones(positions=np.zeros((2, 3)), cell=np.eye(3), cutoff=np.array([1.0, 1.0]))
+    with pytest.raises(TypeError, match="single float cutoff"):
+        lennard_jones(positions=np.zeros((2, 3)), cell=np.eye(3), cutoff={(1, 1): 1.0})
