@@ -28,7 +28,23 @@ METRIC = "pairs/s at 1M atoms rc=5Å (%HBM peak); batched systems/s vs per-syste
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fill_fcc1M.json")
 KERNEL_SOURCE = os.path.join(ROOT, "matscipy-neighbours_amd", "csrc", "mnl_kernels.hip")
-FILL_KERNEL = "k_fill<1, 0, false, 5>"
+FILL_PREFIX = "k_fill<1, 0, false, "
+
+
+def _fill_kernel(chunks):
+    """The float64 uniform-cutoff fill kernel of a given window size."""
+    return f"{FILL_PREFIX}{chunks}>"
+
+
+def _autotune(lib, _lib, plan, sp):
+    """The engine's autotuner on the plan's last count: the launch shape it
+    keeps for this configuration class."""
+    import ctypes
+    split, chunks, ms = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_float(0)
+    _lib.check(lib.mnl_plan_autotune(plan.handle, sp, ctypes.byref(split), ctypes.byref(chunks),
+                                     ctypes.byref(ms)))
+    return {"split": split.value, "chunks": chunks.value, "tuned_count_fill_ms": ms.value,
+            "source": "autotune"}
 
 
 def _source_sha():
@@ -56,10 +72,12 @@ def traffic_summary(fetch_csv, write_csv, cfetch_csv, cwrite_csv, commands):
     copy_bytes = float(1 << 30)
     ff = copy_bytes / (_counter_means(cfetch_csv)["__amd_rocclr_copyBuffer"] * 1024)
     wf = copy_bytes / (_counter_means(cwrite_csv)["__amd_rocclr_copyBuffer"] * 1024)
-    fetch = _counter_means(fetch_csv)[FILL_KERNEL] * 1024
-    write = _counter_means(write_csv)[FILL_KERNEL] * 1024
+    fetches = _counter_means(fetch_csv)
+    kernel = next(k for k in fetches if k.startswith(FILL_PREFIX))
+    fetch = fetches[kernel] * 1024
+    write = _counter_means(write_csv)[kernel] * 1024
     out = {
-        "kernel": FILL_KERNEL,
+        "kernel": kernel,
         "workload": "fcc 64^3 cells (1048576 atoms), rc 5.0 A, ijdDS f64",
         "fetch_size_bytes_raw": fetch, "write_size_bytes_raw": write,
         "copy_calibration": {"fetch_factor": ff, "write_factor": wf,
@@ -110,7 +128,7 @@ def cpu_baseline(w, cell, inv, quantities, reps):
                        f"{med * 1e3:.1f} ms/call; {what}")
 
 
-def batched_leg(lib, _lib, torch, dev, steps, warmup):
+def batched_leg(lib, _lib, torch, dev, steps, warmup, tune=True):
     """1024 uneven periodic systems (8-512 atoms, 0.05 atoms/A^3, tilted
     cells, rc 5 A): one captured graph replay per batch versus one plain
     single-system build (count + fill) per system, outputs compared."""
@@ -136,6 +154,7 @@ def batched_leg(lib, _lib, torch, dev, steps, warmup):
     n0 = ctypes.c_int64(0)
     _lib.check(lib.mnl_plan_batch_count(plan.handle, sp, P_(pos_t), ctypes.byref(n0)))
     P = int(n0.value)
+    shape = _autotune(lib, _lib, plan, sp) if tune else {"source": "built-in"}
 
     def bufs():
         return dict(i=torch.empty(P, dtype=torch.int32, device=dev),
@@ -210,6 +229,7 @@ def batched_leg(lib, _lib, torch, dev, steps, warmup):
         "per_system_systems_per_s": nsys / t_single,
         "speedup": t_single / t_batch,
         "identical_pairs": bool(identical),
+        "launch_shape": shape,
         "how": "batched: one HIP graph replay (count + fill + per-system offsets) timed "
                "with HIP events over the replays; per-system: mnl_plan_count (waits for the "
                "pair total) + mnl_plan_fill per system into pre-allocated slices, wall clock",
@@ -484,6 +504,10 @@ def main():
                     help="skip the other workloads (1000-atom latency, slab, fcc 1M with "
                          "float32 outputs, bcc autograd)")
     ap.add_argument("--reps", type=int, default=64, help="fcc unit cells per edge")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the engine's built-in launch shapes instead of tuning them first "
+                         "(profiling runs: the tuner's candidate launches would enter the "
+                         "per-kernel statistics)")
     ap.add_argument("--leg", choices=["batch"] + list(OTHER_LEGS),
                     help="run only this secondary workload (for profiling it on its own) "
                          "and print its result as one JSON line")
@@ -498,9 +522,9 @@ def main():
     if args.traffic_from:
         traffic_summary(*args.traffic_from, commands=[
             "rocprofv3 --pmc FETCH_SIZE -f csv -d <dir> -o run -- python3 bench.py --steps 3 "
-            "--warmup 1 --no-cpu --no-batch --no-extra",
+            "--warmup 1 --no-cpu --no-batch --no-extra --no-autotune",
             "rocprofv3 --pmc WRITE_SIZE -f csv -d <dir> -o run -- python3 bench.py --steps 3 "
-            "--warmup 1 --no-cpu --no-batch --no-extra",
+            "--warmup 1 --no-cpu --no-batch --no-extra --no-autotune",
             "rocprofv3 --pmc FETCH_SIZE -f csv -d <dir> -o run -- python3 bench.py "
             "--calibrate-copy --steps 3",
             "rocprofv3 --pmc WRITE_SIZE -f csv -d <dir> -o run -- python3 bench.py "
@@ -539,7 +563,8 @@ def main():
     if args.leg:
         _log(f"leg {args.leg}")
         if args.leg == "batch":
-            name, entry = "batched", batched_leg(lib, _lib, torch, dev, args.steps, args.warmup)
+            name, entry = "batched", batched_leg(lib, _lib, torch, dev, args.steps, args.warmup,
+                                                 tune=not args.no_autotune)
         else:
             name, entry = OTHER_LEGS[args.leg](lib, _lib, torch, dev, args.steps, args.warmup,
                                                not args.no_cpu)
@@ -566,6 +591,16 @@ def main():
                                   ctypes.byref(npairs)))
     P = int(npairs.value)
     _log(f"N={N} pairs={P}")
+    if args.no_autotune:
+        # the engine's built-in window: 5 chunks when a home cell's candidates
+        # (27 neighbour cells at the mean density) fit 4 < c/64 <= 5
+        ncells = int(np.prod(np.floor(np.diag(cell) / w["cutoff"])))
+        cand = 27.0 * N / ncells
+        shape = {"source": "built-in", "split": 1, "chunks": 5 if 256 < cand <= 320 else 4}
+    else:
+        shape = _autotune(lib, _lib, plan, sp)
+        _log(f"autotuned launch shape: {shape}")
+    fill_kernel = _fill_kernel(shape["chunks"])
     out_i = torch.empty(P, dtype=torch.int32, device=dev)
     out_j = torch.empty(P, dtype=torch.int32, device=dev)
     out_d = torch.empty(P, dtype=torch.float64, device=dev)
@@ -619,15 +654,19 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             tf = json.load(f)
-        if tf.get("kernel_source_sha256") == _source_sha():
-            traffic = tf["bytes_per_launch"]
-        else:
+        if tf.get("kernel_source_sha256") != _source_sha():
             _log("traffic summary is stale (kernel source changed): traffic = null")
+        elif tf.get("kernel") != fill_kernel:
+            _log(f"traffic summary is for {tf.get('kernel')}, this run used {fill_kernel}: "
+                 "traffic = null")
+        else:
+            traffic = tf["bytes_per_launch"]
 
     batched = None
     if not args.no_batch:
         _log("batched systems leg")
-        batched = batched_leg(lib, _lib, torch, dev, args.steps, args.warmup)
+        batched = batched_leg(lib, _lib, torch, dev, args.steps, args.warmup,
+                              tune=not args.no_autotune)
         _log(f"batched: {batched}")
 
     extra = None
@@ -666,7 +705,7 @@ def main():
         "hbm_frac_pipeline": pipe_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "roofline": {
             "bound": "hbm",
-            "kernel": FILL_KERNEL,
+            "kernel": fill_kernel,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -674,6 +713,7 @@ def main():
             "traffic": traffic,
             "fill_ms": fill_ms,
         },
+        "launch_shape": shape,
         "cpu_baseline": cpu,
         "batched": batched,
         "other_workloads": extra,

@@ -329,6 +329,36 @@ int mnl_plan_pair_backward(mnl_plan *plan, void *stream, int64_t npairs, int32_t
                            const void *grad_absdist, int float32,
                            double *grad_positions, double *grad_cell);
 
+/* ------------------------------------------------------------- autotuner
+ *
+ * Launch shapes of the pair kernels (no counterpart in the reference, which
+ * has no GPU tuning): the home atoms of a cell are dealt to `split` waves
+ * (1..64) and the fill holds `chunks` (4 or 5) chunks of 64 candidates in
+ * registers per window. Any shape gives the same outputs, bit for bit.
+ */
+
+/*
+ * Times the plan's pair kernels (count, scan, and the fill of all five
+ * quantities into internal scratch) on the binned state of its last count or
+ * batch count, for each shape: split 1, 2, 4 and 8 waves per cell (up to
+ * twice the mean atoms per cell) with windows of 4 and 5 chunks; one
+ * warm-up, then the fastest of three runs. The fastest shape is kept in a
+ * process-wide cache keyed by the configuration class (systems, atoms,
+ * cells or hash slots, grid backend, cutoff form), which every later build
+ * of that class uses. Waits for the stream, allocates 52 bytes of scratch per
+ * pair (at most 2^30 pairs) and is refused under graph capture. split,
+ * chunks and best_ms (may be NULL) receive the choice and its time.
+ */
+int mnl_plan_autotune(mnl_plan *plan, void *stream, int32_t *split, int32_t *chunks,
+                      float *best_ms);
+
+/* Forget every tuned shape: builds return to the built-in choice. */
+void mnl_autotune_clear(void);
+
+/* Force the plan's launch shape (split 1..64, chunks 4 or 5, or 0 for the
+   built-in choice; split 0 restores the tuned or built-in shape). */
+int mnl_plan_set_launch_shape(mnl_plan *plan, int32_t split, int32_t chunks);
+
 /* ------------------------------------------------------- fused consumer */
 
 /*

@@ -22,8 +22,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "mnl.h"
@@ -123,7 +126,7 @@ struct mnl_plan {
     DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, scount, slot,
         counts, order_stats,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
-        htypes, cell_partial, tri_m, tri_cnt, tri_off, tri_vidx, tri_flag, lj;
+        htypes, cell_partial, tri_m, tri_cnt, tri_off, tri_vidx, tri_flag, lj, tune_out;
     int64_t *host_total = nullptr; /* pinned */
     /* state of the last count */
     bool counted = false;
@@ -136,6 +139,9 @@ struct mnl_plan {
     int64_t npairs = 0;
     int32_t ncut = 0;
     int32_t fill_chunks = 4; /* fill window (chunks of 64 candidates) */
+    /* launch shape set by mnl_plan_set_launch_shape or the autotuner while it
+       times a candidate (0: the tuned or built-in choice) */
+    int32_t force_split = 0, force_chunks = 0;
     hipStream_t stream = nullptr;
     /* host copy of the per-type table on the device */
     std::vector<double> pt_host;
@@ -149,6 +155,26 @@ struct mnl_plan {
 };
 
 namespace {
+
+/* Launch shapes chosen by the autotuner, per configuration class: the same
+   systems, atoms, cells (or hash slots), grid backend and cutoff form. The
+   shape changes how work is dealt to waves, never the outputs. */
+struct TuneKey {
+    int32_t nsys, nat, cells, backend, cut;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(nsys, nat, cells, backend, cut) <
+               std::tie(o.nsys, o.nat, o.cells, o.backend, o.cut);
+    }
+};
+struct TuneShape {
+    int32_t split, chunks;
+};
+std::mutex g_tune_mutex;
+std::map<TuneKey, TuneShape> g_tune;
+
+TuneKey tune_key(const mnl_plan *p) {
+    return TuneKey{p->nsys, p->nat, p->home_cells, p->backend, p->cut};
+}
 
 /* The pair-kernel arguments of the plan's last count. */
 PairArgs pair_args(const mnl_plan *p) {
@@ -176,6 +202,18 @@ PairArgs pair_args(const mnl_plan *p) {
         const int64_t want = (4096 + cells - 1) / cells;
         const int64_t per_cell = (static_cast<int64_t>(p->nat) + cells - 1) / cells;
         a.split = static_cast<int32_t>(std::max<int64_t>(1, std::min(want, per_cell)));
+    }
+    /* a forced shape, else a tuned one for this configuration class */
+    if (p->force_split > 0) {
+        a.split = p->force_split;
+        if (p->force_chunks > 0) a.fill_chunks = p->force_chunks;
+    } else {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tune.find(tune_key(p));
+        if (it != g_tune.end()) {
+            a.split = it->second.split;
+            a.fill_chunks = it->second.chunks;
+        }
     }
     a.atom_base = p->atom_base.as<int32_t>();
     a.nat = p->nat;
@@ -672,7 +710,7 @@ void mnl_plan_destroy(mnl_plan *p) {
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
                       &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
                       &p->cell_partial, &p->tri_m,   &p->tri_cnt,  &p->tri_off,
-                      &p->tri_vidx,  &p->tri_flag,  &p->lj};
+                      &p->tri_vidx,  &p->tri_flag,  &p->lj,     &p->tune_out};
     for (DevBuf *b : bufs) b->release();
     if (p->host_total) (void)hipHostFree(p->host_total);
     delete p;
@@ -1172,6 +1210,111 @@ int mnl_plan_lj(mnl_plan *p, void *stream, const double origin[3], const double 
     MNL_HIP(launch_lj(a, p->backend == MNL_BACKEND_HASHED, epsilon, sigma,
                       p->slot.as<int32_t>(), p->lj.as<double>(), forces, energy, virial, st));
     return MNL_SUCCESS;
+}
+
+int mnl_plan_set_launch_shape(mnl_plan *p, int32_t split, int32_t chunks) {
+    if (!p) return fail("plan is NULL");
+    if (split < 0 || split > 64) return fail("split must be 0 (automatic) or 1..64");
+    if (chunks != 0 && chunks != 4 && chunks != 5) return fail("chunks must be 0, 4 or 5");
+    p->force_split = split;
+    p->force_chunks = split > 0 ? chunks : 0;
+    return MNL_SUCCESS;
+}
+
+int mnl_plan_autotune(mnl_plan *p, void *stream, int32_t *split, int32_t *chunks,
+                      float *best_ms) {
+    if (!p) return fail("plan is NULL");
+    if (!p->counted) return fail("mnl_plan_autotune needs a count (or batch count) first");
+    DeviceScope ds(p->device);
+    if (!ds.ok) return fail("cannot switch to the plan's device");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    MNL_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) return fail("mnl_plan_autotune cannot run under capture");
+    int64_t P = 0;
+    if (read_total(p, st, &P) != MNL_SUCCESS) return MNL_ERROR;
+    constexpr int64_t kTuneMaxPairs = int64_t(1) << 30;
+    if (P > kTuneMaxPairs) return fail("too many pairs to autotune (limit 2^30)");
+    /* scratch outputs i, j, D, d, S: 52 bytes per pair, 8-byte aligned parts */
+    const size_t n = static_cast<size_t>(std::max<int64_t>(P, 1));
+    MNL_HIP(hipStreamSynchronize(st));
+    MNL_HIP(p->tune_out.ensure(52 * n));
+    char *base = p->tune_out.as<char>();
+    int32_t *oi = reinterpret_cast<int32_t *>(base);
+    int32_t *oj = oi + n;
+    double *oD = reinterpret_cast<double *>(base + 8 * n);
+    double *od = oD + 3 * n;
+    int32_t *oS = reinterpret_cast<int32_t *>(base + 40 * n);
+    /* candidates: split over up to twice the mean atoms per cell */
+    const int64_t cells = std::max<int64_t>(p->home_cells, 1);
+    const int64_t per_cell = (static_cast<int64_t>(p->nat) + cells - 1) / cells;
+    std::vector<TuneShape> cand;
+    for (int32_t sp : {1, 2, 4, 8}) {
+        if (sp > 1 && sp > 2 * per_cell) continue;
+        for (int32_t ch : {4, 5}) cand.push_back(TuneShape{sp, ch});
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MNL_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail("hipEventCreate failed");
+    }
+    const int32_t saved_split = p->force_split, saved_chunks = p->force_chunks;
+    TuneShape best{1, 4};
+    float best_t = std::numeric_limits<float>::infinity();
+    int status = MNL_SUCCESS;
+    for (const TuneShape &c : cand) {
+        p->force_split = c.split;
+        p->force_chunks = c.chunks;
+        float t = std::numeric_limits<float>::infinity();
+        for (int rep = 0; rep < 4 && status == MNL_SUCCESS; rep++) {
+            /* one warm-up, then the fastest of three */
+            hipError_t e = hipEventRecord(e0, st);
+            if (e == hipSuccess && p->nat > 0)
+                e = launch_fill_i32(p->scount.as<int32_t>(), p->nat, 0, st);
+            if (e != hipSuccess) {
+                status = fail(std::string("autotune: ") + hipGetErrorString(e));
+                break;
+            }
+            if (enqueue_count_scan(p, st, nullptr) != MNL_SUCCESS ||
+                enqueue_fill(p, st, 1, oi, oj, oD, od, oS) != MNL_SUCCESS) {
+                status = MNL_ERROR;
+                break;
+            }
+            float ms = 0.0f;
+            e = hipEventRecord(e1, st);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) {
+                status = fail(std::string("autotune: ") + hipGetErrorString(e));
+                break;
+            }
+            if (rep > 0) t = std::min(t, ms);
+        }
+        if (status != MNL_SUCCESS) break;
+        if (t < best_t) {
+            best_t = t;
+            best = c;
+        }
+    }
+    p->force_split = saved_split;
+    p->force_chunks = saved_chunks;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (status != MNL_SUCCESS) return status;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_tune[tune_key(p)] = best;
+    }
+    if (split) *split = best.split;
+    if (chunks) *chunks = best.chunks;
+    if (best_ms) *best_ms = best_t;
+    return MNL_SUCCESS;
+}
+
+void mnl_autotune_clear(void) {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tune.clear();
 }
 
 int mnl_first_neighbours(void *stream, int32_t n, int64_t nn, const int32_t *i, int32_t *seed) {

@@ -9,6 +9,8 @@ bits of D and d are identical to the reference CPU path.
 from .autograd import neighbour_list_autograd
 from .neighbours import (
     DLPackTensor,
+    autotune,
+    autotune_batch,
     coordination,
     first_neighbours,
     get_jump_indicies,
@@ -31,5 +33,7 @@ __all__ = [
     "mic",
     "coordination",
     "lennard_jones",
+    "autotune",
+    "autotune_batch",
     "DLPackTensor",
 ]

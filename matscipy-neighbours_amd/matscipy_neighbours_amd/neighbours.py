@@ -32,6 +32,9 @@ __all__ = [
     "get_jump_indicies",
     "triplet_list",
     "mic",
+    "lennard_jones",
+    "autotune",
+    "autotune_batch",
     "DLPackTensor",
 ]
 
@@ -575,6 +578,65 @@ def lennard_jones(atoms=None, cutoff=None, *, epsilon=1.0, sigma=1.0, positions=
         return float(energy.item()), forces.cpu().numpy(), virial.cpu().numpy()
     e, f, w = _export([energy.reshape(()), forces, virial], want_dev, array_namespace)
     return e, f, w
+
+
+def _tune_plan(dev):
+    torch = _torch()
+    lib = _lib.load()
+    idx = dev.index if dev.index is not None else torch.cuda.current_device()
+    plan = _lib.plan_for(idx)
+    split, chunks, ms = _lib.ctypes.c_int32(0), _lib.ctypes.c_int32(0), _lib.ctypes.c_float(0)
+    with plan.lock, torch.cuda.device(idx):
+        sp = _lib.ctypes.c_void_p(torch.cuda.current_stream(idx).cuda_stream)
+        _lib.check(lib.mnl_plan_autotune(plan.handle, sp, _lib.ctypes.byref(split),
+                                         _lib.ctypes.byref(chunks), _lib.ctypes.byref(ms)))
+    return {"split": split.value, "chunks": chunks.value, "ms": ms.value}
+
+
+def autotune(atoms=None, cutoff=None, *, positions=None, cell=None, pbc=None, numbers=None,
+             cell_origin=None, device=None):
+    """Tune the pair kernels' launch shape for this system's configuration
+    class (atom count, cells, grid backend, cutoff form) on the GPU: the
+    fastest candidate shape is kept for the process and used by every later
+    build of the same class. Outputs never depend on the shape. Returns
+    ``{"split", "chunks", "ms"}``. No counterpart in the reference."""
+    if cutoff is None:
+        raise ValueError("Please provide a value for the cutoff radius.")
+    pos_t, co, ce, inv, pb, nums, on_device, want_dev = _prepare(
+        atoms, positions, cell, pbc, numbers, cell_origin, device)
+    rc, nums = _resolve_cutoff(cutoff, nums)
+    b = _Build(pos_t, co, ce, inv, pb, rc, nums)
+    b.run(0)
+    return _tune_plan(b.dev)
+
+
+def autotune_batch(positions, cells, pbc, cutoff, natoms, *, cell_origins=None, device=None):
+    """``autotune`` for the configuration class of a batch (arguments as
+    ``neighbour_list_batch``)."""
+    nsys, offs, co, ce, inv, pb, rc = _batch_geometry(cells, pbc, cutoff, natoms, cell_origins)
+    on_device = _is_on_device(positions)
+    want_dev, device_id = _resolve_device(device, on_device)
+    _lib.require_gpu()
+    torch = _torch()
+    if on_device:
+        pos_t = _device_positions(positions)
+    else:
+        dev = device_id if device_id >= 0 else torch.cuda.current_device()
+        pos_t = torch.from_numpy(np.ascontiguousarray(positions, dtype=float)).to(f"cuda:{dev}")
+    if pos_t.shape[0] != int(offs[-1]):
+        raise ValueError("positions must hold sum(natoms) rows")
+    lib = _lib.load()
+    idx = pos_t.device.index if pos_t.device.index is not None else torch.cuda.current_device()
+    plan = _lib.plan_for(idx)
+    with plan.lock, torch.cuda.device(idx):
+        sp = _lib.ctypes.c_void_p(torch.cuda.current_stream(idx).cuda_stream)
+        _lib.check(lib.mnl_plan_set_batch(plan.handle, sp, nsys, _lib.ptr(offs), _lib.ptr(co),
+                                          _lib.ptr(ce), _lib.ptr(inv), _lib.ptr(pb),
+                                          _lib.ptr(rc)))
+        npairs = _lib.ctypes.c_int64(0)
+        _lib.check(lib.mnl_plan_batch_count(plan.handle, sp, _lib.ptr(pos_t),
+                                            _lib.ctypes.byref(npairs)))
+    return _tune_plan(pos_t.device)
 
 
 def _int32_on_device(x):
