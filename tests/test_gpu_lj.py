@@ -191,3 +191,17 @@ def test_lj_virial_is_minus_strain_derivative(gpu):
         # the two halves of a symmetric off-diagonal e adding up
         want = -(es[0] - es[1]) / (2 * h)
         assert abs(w[a, b] - want) <= 1e-5 * (1.0 + abs(w[a, b])), (a, b, w[a, b], want)
+
+
+def test_lj_sparse_box_hashed_grid(gpu):
+    """A sparse cloud in a 600 A box at rc 3 (200^3 cells, more than the dense
+    budget allows): the build takes the hashed grid, and the fused sums still
+    match the oracle list's."""
+    from matscipy_neighbours_amd import _lib
+    rng = np.random.default_rng(86)
+    cell = np.eye(3) * 600.0
+    centres = rng.uniform(50.0, 550.0, (30, 3))
+    pos = (centres[:, None, :] + rng.normal(0.0, 2.5, (30, 40, 3))).reshape(-1, 3)
+    _compare(pos, cell, False, 3.0, 0.9, 1.2)
+    plan = _lib.plan_for(0)
+    assert _lib.load().mnl_plan_backend(plan.handle) == 1
