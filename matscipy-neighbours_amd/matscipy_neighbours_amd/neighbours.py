@@ -260,7 +260,11 @@ class _Build:
         self.pos = positions_t
         self.backend = backend
 
-    def run(self, flags, want_counts=False, f32=False):
+    def run(self, flags, want_counts=False, f32=False, packed=False):
+        """Count, then fill the quantities in ``flags``. With ``packed`` the
+        outputs are views of one device buffer (``out["_packed"]``: D and d
+        first, then i, j, S, each part aligned to its element size), so that a
+        host result needs a single transfer."""
         torch = self.torch
         lib = _lib.load()
         idx = self.dev.index if self.dev.index is not None else torch.cuda.current_device()
@@ -278,16 +282,21 @@ class _Build:
             P = int(npairs.value)
             fdt = torch.float32 if f32 else torch.float64
             kw = dict(device=self.dev)
-            if flags & 1:
-                out["i"] = torch.empty(P, dtype=torch.int32, **kw)
-            if flags & 2:
-                out["j"] = torch.empty(P, dtype=torch.int32, **kw)
-            if flags & 4:
-                out["D"] = torch.empty((P, 3), dtype=fdt, **kw)
-            if flags & 8:
-                out["d"] = torch.empty(P, dtype=fdt, **kw)
-            if flags & 16:
-                out["S"] = torch.empty((P, 3), dtype=torch.int32, **kw)
+            parts = [(q, shape, dt) for q, bit, shape, dt in (
+                ("D", 4, (P, 3), fdt), ("d", 8, (P,), fdt), ("i", 1, (P,), torch.int32),
+                ("j", 2, (P,), torch.int32), ("S", 16, (P, 3), torch.int32)) if flags & bit]
+            if packed:
+                sizes = [int(np.prod(shape)) * (torch.empty((), dtype=dt).element_size())
+                         for _, shape, dt in parts]
+                buf = torch.empty(sum(sizes), dtype=torch.uint8, **kw)
+                off = 0
+                for (q, shape, dt), nb in zip(parts, sizes):
+                    out[q] = buf[off:off + nb].view(dt).view(shape)
+                    off += nb
+                out["_packed"] = buf
+            else:
+                for q, shape, dt in parts:
+                    out[q] = torch.empty(shape, dtype=dt, **kw)
             if flags:
                 fill = lib.mnl_plan_fill_f32 if f32 else lib.mnl_plan_fill
                 _lib.check(fill(plan.handle, _lib.ctypes.c_void_p(stream),
@@ -377,10 +386,33 @@ def neighbour_list(quantities, atoms=None, cutoff=None, *, positions=None, cell=
         atoms, positions, cell, pbc, numbers, cell_origin, device)
     rc, nums = _resolve_cutoff(cutoff, nums)
     b = _Build(pos_t, co, ce, inv, pb, rc, nums)
-    out = b.run(flags, f32=f32)
-    tensors = [out[q] for q in quantities]
-    arrays = _export(tensors, want_dev, array_namespace)
+    host_out = not want_dev and array_namespace is None
+    out = b.run(flags, f32=f32, packed=host_out)
+    if host_out:
+        arrays = _packed_to_host(out, quantities)
+    else:
+        arrays = _export([out[q] for q in quantities], want_dev, array_namespace)
     return arrays[0] if len(quantities) == 1 else tuple(arrays)
+
+
+def _packed_to_host(out, quantities):
+    """numpy arrays of a packed result: one transfer of the whole device
+    buffer into pinned host memory, then a view per quantity."""
+    torch = _torch()
+    buf = out["_packed"]
+    host = torch.empty(buf.numel(), dtype=torch.uint8, pin_memory=True)
+    host.copy_(buf, non_blocking=True)
+    torch.cuda.current_stream(buf.device).synchronize()
+    raw = host.numpy()
+    base = buf.data_ptr()
+    res = []
+    for q in quantities:
+        t = out[q]
+        off = t.data_ptr() - base if t.numel() else 0
+        nb = t.numel() * t.element_size()
+        dt = np.dtype(str(t.dtype).replace("torch.", ""))
+        res.append(raw[off:off + nb].view(dt).reshape(tuple(t.shape)))
+    return res
 
 
 def _batch_geometry(cells, pbc, cutoff, natoms, cell_origins):
