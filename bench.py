@@ -475,8 +475,65 @@ def leg_lj(lib, _lib, torch, dev, steps, warmup, with_cpu):
                "over the i-sorted rows; HIP events over the steps"}
 
 
+def leg_alloc(lib, _lib, torch, dev, steps, warmup, with_cpu):
+    """The fcc 1M ijdDS build with the output allocation inside the timed
+    region: the one-shot C entry point (a device result owning its buffers,
+    allocated per call and freed after it) and the Python neighbour_list with
+    device positions (outputs from torch's caching allocator). Host clock,
+    median over the calls."""
+    import ctypes
+
+    import numpy as np
+
+    from matscipy_neighbours_amd import neighbour_list, workloads
+    w = workloads.fcc_bulk()
+    N = len(w["positions"])
+    cell = np.ascontiguousarray(w["cell"], dtype=np.float64)
+    inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+    pbc = np.ones(3, np.int32)
+    co = np.zeros(3)
+    P_ = _lib.ptr
+    pos = torch.from_numpy(w["positions"]).to(dev)
+    res = ctypes.c_void_p()
+    npairs = []
+
+    def oneshot():
+        _lib.check(lib.mnl_neighbour_list_device(31, P_(co), P_(cell), P_(inv), P_(pbc), N,
+                                                 P_(pos), 1, w["cutoff"], None, None, 0, None,
+                                                 dev.index or 0, ctypes.byref(res)))
+        torch.cuda.synchronize()
+        npairs.append(lib.mnl_result_npairs(res))
+        lib.mnl_result_free(res)
+
+    def pyapi():
+        out = neighbour_list("ijdDS", positions=pos, cell=cell, pbc=True, cutoff=w["cutoff"])
+        torch.cuda.synchronize()
+        del out
+
+    def host_ms(fn):
+        for _ in range(warmup):
+            fn()
+        t = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            fn()
+            t.append(time.perf_counter() - t0)
+        return float(np.median(t)) * 1e3
+
+    ms_one = host_ms(oneshot)
+    ms_py = host_ms(pyapi)
+    P = int(npairs[-1])
+    return "fcc_1M_with_allocation", {
+        "atoms": N, "pairs": P, "oneshot_ms": ms_one, "python_api_ms": ms_py,
+        "oneshot_pairs_per_s": P / (ms_one * 1e-3),
+        "how": "one-shot: mnl_neighbour_list_device (count, host read of the total, output "
+               "allocation, fill) + mnl_result_free per call; Python: neighbour_list('ijdDS') "
+               "with device positions; host clock around each call including a device "
+               "synchronisation, median"}
+
+
 OTHER_LEGS = {"latency": leg_latency, "slab": leg_slab, "fcc_f32": leg_fcc_f32,
-              "autograd": leg_autograd, "lj": leg_lj}
+              "autograd": leg_autograd, "lj": leg_lj, "alloc": leg_alloc}
 
 
 def other_workloads(lib, _lib, torch, dev, steps, warmup, with_cpu):

@@ -24,5 +24,5 @@ def test_more_than_one_gpu_is_refused():
 def test_legs_listed():
     r = _bench("--help")
     assert r.returncode == 0
-    for leg in ("batch", "latency", "slab", "fcc_f32", "autograd", "lj"):
+    for leg in ("batch", "latency", "slab", "fcc_f32", "autograd", "lj", "alloc"):
         assert leg in r.stdout
