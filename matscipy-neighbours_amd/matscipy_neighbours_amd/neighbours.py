@@ -395,14 +395,23 @@ def neighbour_list(quantities, atoms=None, cutoff=None, *, positions=None, cell=
     return arrays[0] if len(quantities) == 1 else tuple(arrays)
 
 
+# Results up to this size go through pinned host memory (torch's caching
+# host allocator); larger ones through one pageable copy, since page-locking
+# gigabytes the first time costs more than it saves.
+_PINNED_MAX_BYTES = 256 << 20
+
+
 def _packed_to_host(out, quantities):
     """numpy arrays of a packed result: one transfer of the whole device
-    buffer into pinned host memory, then a view per quantity."""
+    buffer to the host, then a view per quantity."""
     torch = _torch()
     buf = out["_packed"]
-    host = torch.empty(buf.numel(), dtype=torch.uint8, pin_memory=True)
-    host.copy_(buf, non_blocking=True)
-    torch.cuda.current_stream(buf.device).synchronize()
+    if buf.numel() <= _PINNED_MAX_BYTES:
+        host = torch.empty(buf.numel(), dtype=torch.uint8, pin_memory=True)
+        host.copy_(buf, non_blocking=True)
+        torch.cuda.current_stream(buf.device).synchronize()
+    else:
+        host = buf.cpu()
     raw = host.numpy()
     base = buf.data_ptr()
     res = []

@@ -211,3 +211,18 @@ def test_beyond_int32_pairs(gpu):
 
     with pytest.raises(RuntimeError, match="2\\^31"):
         nb.first_neighbours(n, i)
+
+
+def test_full_size_host_arrays(gpu):
+    """fcc 1M through the public API with numpy in and out: the 2.4 GB result
+    comes back in one transfer (pageable above 256 MiB), bit for bit against
+    the oracle."""
+    from matscipy_neighbours_amd import neighbour_list, workloads
+    w = workloads.fcc_bulk()
+    got = neighbour_list("ijdDS", positions=w["positions"], cell=w["cell"], pbc=w["pbc"],
+                         cutoff=w["cutoff"])
+    assert all(isinstance(a, np.ndarray) for a in got)
+    cell = np.ascontiguousarray(w["cell"], dtype=float)
+    inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+    ref = _oracle(w, cell, inv)
+    assert_same_pairs(dict(zip("ijdDS", got)), ref, msg="fcc 1M host arrays")
