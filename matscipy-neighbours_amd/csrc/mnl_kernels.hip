@@ -4,7 +4,8 @@
  * Pipeline of one build (plain path = a batch of one system):
  *   bin         raw cell per atom, fold, per-cell histogram with an atomic
  *               ticket                                     (1 thread / atom)
- *   scan        cell counts -> cell slice starts           (3-phase scan)
+ *   scan        cell counts -> cell slice starts  (reduce + apply; a spine
+ *               pass beyond 1024 tiles; the apply alone for one tile)
  *   scatter     atoms into their cell slice at the ticket  (1 thread / atom)
  *   rank+gather stable order inside each cell (ascending atom index, the
  *               reference's serial counting sort) + gather of the
@@ -14,6 +15,13 @@
  *   fill        i, j, D, d, S at the offsets: the count's traversal again,
  *               accepted pairs compacted in LDS and written 64 at a time
  *                                                          (1 wave / cell)
+ *
+ * A small single system (at most 4096 atoms and 4096 cells) is binned by one
+ * block instead (k_bin_small: parameters, histogram, scan, member lists and
+ * stable ranks in LDS). Beside the list: the dense neighbour matrix (the
+ * fill's traversal into fixed rows), the fused Lennard-Jones consumer (k_lj:
+ * forces, energy and virial from the traversal, no list written) and the
+ * backward of D and d (k_pair_backward).
  *
  * Floating point: compiled with contraction off and the reference's
  * operation order everywhere a double is produced, so the cell index, the

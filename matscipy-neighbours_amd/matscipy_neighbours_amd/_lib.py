@@ -148,12 +148,14 @@ def require_gpu():
 
 
 def ptr(x):
-    """Raw address of a numpy array or torch tensor (None -> NULL)."""
+    """Raw address of a numpy array or torch tensor as an int (None -> NULL);
+    the bound entry points declare their pointers c_void_p, which takes it
+    as is."""
     if x is None:
         return None
     if hasattr(x, "data_ptr"):
-        return ctypes.c_void_p(x.data_ptr())
-    return ctypes.c_void_p(x.ctypes.data)
+        return x.data_ptr()
+    return x.__array_interface__["data"][0]
 
 
 class Plan:
