@@ -478,9 +478,11 @@ def leg_lj(lib, _lib, torch, dev, steps, warmup, with_cpu):
 def leg_alloc(lib, _lib, torch, dev, steps, warmup, with_cpu):
     """The fcc 1M ijdDS build with the output allocation inside the timed
     region: the one-shot C entry point (a device result owning its buffers,
-    allocated per call and freed after it) and the Python neighbour_list with
-    device positions (outputs from torch's caching allocator). Host clock,
-    median over the calls."""
+    taken per call from the library's kept blocks and given back by
+    mnl_result_free), on the same positions every call and on four copies
+    jittered by 0.01 A in turn, whose pair totals differ as an MD loop's do;
+    and the Python neighbour_list with device positions (outputs from torch's
+    caching allocator). Host clock, median over the calls."""
     import ctypes
 
     import numpy as np
@@ -494,16 +496,28 @@ def leg_alloc(lib, _lib, torch, dev, steps, warmup, with_cpu):
     co = np.zeros(3)
     P_ = _lib.ptr
     pos = torch.from_numpy(w["positions"]).to(dev)
+    rng = np.random.default_rng(11)
+    jittered = [torch.from_numpy(w["positions"] + rng.normal(0.0, 0.01, (N, 3))).to(dev)
+                for _ in range(4)]
     res = ctypes.c_void_p()
     npairs = []
 
-    def oneshot():
+    def oneshot_on(p):
         _lib.check(lib.mnl_neighbour_list_device(31, P_(co), P_(cell), P_(inv), P_(pbc), N,
-                                                 P_(pos), 1, w["cutoff"], None, None, 0, None,
+                                                 P_(p), 1, w["cutoff"], None, None, 0, None,
                                                  dev.index or 0, ctypes.byref(res)))
         torch.cuda.synchronize()
         npairs.append(lib.mnl_result_npairs(res))
         lib.mnl_result_free(res)
+
+    def oneshot():
+        oneshot_on(pos)
+
+    calls = [0]
+
+    def oneshot_jittered():
+        oneshot_on(jittered[calls[0] % 4])
+        calls[0] += 1
 
     def pyapi():
         out = neighbour_list("ijdDS", positions=pos, cell=cell, pbc=True, cutoff=w["cutoff"])
@@ -521,15 +535,21 @@ def leg_alloc(lib, _lib, torch, dev, steps, warmup, with_cpu):
         return float(np.median(t)) * 1e3
 
     ms_one = host_ms(oneshot)
-    ms_py = host_ms(pyapi)
     P = int(npairs[-1])
+    npairs.clear()
+    ms_jit = host_ms(oneshot_jittered)
+    P_jit = sorted(set(int(x) for x in npairs[-4:]))
+    lib.mnl_release_cached_memory()
+    ms_py = host_ms(pyapi)
     return "fcc_1M_with_allocation", {
-        "atoms": N, "pairs": P, "oneshot_ms": ms_one, "python_api_ms": ms_py,
+        "atoms": N, "pairs": P, "oneshot_ms": ms_one, "oneshot_jittered_ms": ms_jit,
+        "jittered_pairs": P_jit, "python_api_ms": ms_py,
         "oneshot_pairs_per_s": P / (ms_one * 1e-3),
         "how": "one-shot: mnl_neighbour_list_device (count, host read of the total, output "
-               "allocation, fill) + mnl_result_free per call; Python: neighbour_list('ijdDS') "
-               "with device positions; host clock around each call including a device "
-               "synchronisation, median"}
+               "allocation, fill) + mnl_result_free per call, on the same positions and on "
+               "four copies jittered by 0.01 A (normal) in turn; Python: "
+               "neighbour_list('ijdDS') with device positions; host clock around each call "
+               "including a device synchronisation, median"}
 
 
 OTHER_LEGS = {"latency": leg_latency, "slab": leg_slab, "fcc_f32": leg_fcc_f32,

@@ -129,8 +129,17 @@ int mnl_result_copy_to_host(const mnl_result *res, char quantity, void *dst);
  */
 DLManagedTensor *mnl_result_export_dlpack(mnl_result *res, char quantity);
 
-/* Free a result and the device buffers it still owns. NULL is a no-op. */
+/* Free a result and the device buffers it still owns. NULL is a no-op.
+   Output buffers are allocated at size classes (rounded up to 1/8 of their
+   highest power of two, at least 64 KiB). A freed buffer (here or by an
+   exported tensor's DLPack deleter) is kept, once the device has finished
+   the work queued before the free, for the next result whose buffer has its
+   class on its device; at most 16 GiB is kept in all. */
 void mnl_result_free(mnl_result *res);
+
+/* Return every kept result buffer to the device (no counterpart in the
+   reference, which keeps none). */
+void mnl_release_cached_memory(void);
 
 /* ------------------------------------------------------------------------
  * Plan API: reusable device workspace, caller-owned outputs, explicit stream.

@@ -547,11 +547,82 @@ size_t bytes_of(const mnl_result *r, int slot) {
     }
 }
 
+/* Output buffers of one-shot results. Each is allocated at its size class
+   (block_bytes: rounded up to 1/8 of its highest power of two, at least
+   64 KiB), so lists whose lengths differ by a little, as an MD loop's do from
+   step to step, share blocks. A freed buffer is kept, once the device has
+   finished all work queued before the free (hipFree waits for it too), and
+   handed to the next result whose buffer has the same class on the same
+   device; at most kBlockCacheMax bytes are kept. The reference allocates and
+   frees every output per call (neighbour_list_gpu.cc:264-305); with buffers
+   of gigabytes that costs more than the build. */
+struct CachedBlock {
+    int device;
+    size_t bytes;
+    void *ptr;
+};
+std::mutex g_blocks_mutex;
+std::vector<CachedBlock> g_blocks;
+size_t g_blocks_bytes = 0;
+constexpr size_t kBlockCacheMax = size_t(16) << 30;
+
+size_t block_bytes(size_t bytes) {
+    size_t top = 1;
+    while (top <= bytes / 2) top *= 2;
+    const size_t g = std::max<size_t>(size_t(64) << 10, top / 8);
+    return (bytes + g - 1) / g * g;
+}
+
+hipError_t result_malloc(int device, void **p, size_t need) {
+    const size_t bytes = block_bytes(need);
+    {
+        std::lock_guard<std::mutex> lock(g_blocks_mutex);
+        for (size_t k = 0; k < g_blocks.size(); k++) {
+            if (g_blocks[k].device == device && g_blocks[k].bytes == bytes) {
+                *p = g_blocks[k].ptr;
+                g_blocks_bytes -= bytes;
+                g_blocks.erase(g_blocks.begin() + static_cast<std::ptrdiff_t>(k));
+                return hipSuccess;
+            }
+        }
+    }
+    return hipMalloc(p, bytes);
+}
+
+/* With the device current; `need` as passed to result_malloc. */
+void result_free(int device, void *p, size_t need) {
+    if (!p) return;
+    const size_t bytes = need > 0 ? block_bytes(need) : 0;
+    if (bytes > 0 && bytes <= kBlockCacheMax && hipDeviceSynchronize() == hipSuccess) {
+        std::vector<void *> evict;
+        {
+            std::lock_guard<std::mutex> lock(g_blocks_mutex);
+            /* oldest blocks of this device make room first */
+            for (size_t k = 0; k < g_blocks.size() && g_blocks_bytes + bytes > kBlockCacheMax;) {
+                if (g_blocks[k].device == device) {
+                    evict.push_back(g_blocks[k].ptr);
+                    g_blocks_bytes -= g_blocks[k].bytes;
+                    g_blocks.erase(g_blocks.begin() + static_cast<std::ptrdiff_t>(k));
+                } else {
+                    k++;
+                }
+            }
+            if (g_blocks_bytes + bytes <= kBlockCacheMax) {
+                g_blocks.push_back(CachedBlock{device, bytes, p});
+                g_blocks_bytes += bytes;
+                p = nullptr;
+            }
+        }
+        for (void *e : evict) (void)hipFree(e);
+    }
+    if (p) (void)hipFree(p);
+}
+
 void free_result_buffers(mnl_result *r) {
     DeviceScope ds(r->device);
-    for (void *&b : r->buf) {
-        if (b) (void)hipFree(b);
-        b = nullptr;
+    for (int s = 0; s < 6; s++) {
+        result_free(r->device, r->buf[s], bytes_of(r, s));
+        r->buf[s] = nullptr;
     }
 }
 
@@ -616,7 +687,7 @@ int one_shot(int quantities, bool want_pairs, const double origin[3], const doub
     auto alloc = [&](int slot) -> int {
         const size_t b = bytes_of(res, slot);
         if (b == 0) return MNL_SUCCESS;
-        hipError_t e = hipMalloc(&res->buf[slot], b);
+        hipError_t e = result_malloc(dev, &res->buf[slot], b);
         if (e != hipSuccess) {
             res->buf[slot] = nullptr;
             return fail(std::string("hipMalloc of an output failed: ") + hipGetErrorString(e));
@@ -650,6 +721,7 @@ int one_shot(int quantities, bool want_pairs, const double origin[3], const doub
 
 struct DLCtx {
     void *ptr;
+    size_t bytes;
     int device;
     int64_t shape[2];
 };
@@ -659,7 +731,7 @@ void dl_deleter(DLManagedTensor *t) {
     auto *c = static_cast<DLCtx *>(t->manager_ctx);
     if (c) {
         DeviceScope ds(c->device);
-        if (c->ptr) (void)hipFree(c->ptr);
+        result_free(c->device, c->ptr, c->bytes);
         delete c;
     }
     delete t;
@@ -866,6 +938,7 @@ DLManagedTensor *mnl_result_export_dlpack(mnl_result *r, char q) {
     }
     auto *ctx = new DLCtx();
     ctx->ptr = r->buf[s];
+    ctx->bytes = bytes_of(r, s);
     ctx->device = r->device;
     r->buf[s] = nullptr;
     if (s < 5) r->quantities &= ~(1 << s);
@@ -887,6 +960,19 @@ DLManagedTensor *mnl_result_export_dlpack(mnl_result *r, char q) {
     t->manager_ctx = ctx;
     t->deleter = dl_deleter;
     return t;
+}
+
+void mnl_release_cached_memory(void) {
+    std::vector<CachedBlock> blocks;
+    {
+        std::lock_guard<std::mutex> lock(g_blocks_mutex);
+        blocks.swap(g_blocks);
+        g_blocks_bytes = 0;
+    }
+    for (const CachedBlock &b : blocks) {
+        DeviceScope ds(b.device);
+        (void)hipFree(b.ptr);
+    }
 }
 
 void mnl_result_free(mnl_result *r) {

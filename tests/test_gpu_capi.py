@@ -133,3 +133,78 @@ def test_zero_volume_error(gpu):
     assert st == -1
     assert L.mnl_last_error() == b"Zero cell volume."
     assert not res.value
+
+
+def _block_bytes(b):
+    """The library's size classes for one-shot buffers: b rounded up to 1/8
+    of its highest power of two, at least 64 KiB."""
+    top = 1
+    while top <= b // 2:
+        top *= 2
+    g = max(64 << 10, top // 8)
+    return (b + g - 1) // g * g
+
+
+def test_result_buffers_reused(gpu):
+    """Freed one-shot outputs (by mnl_result_free or by an exported tensor's
+    DLPack deleter) serve the next result whose buffers fall in the same size
+    classes: the same device buffers come back, for the same input and for a
+    jittered copy with 46 more pairs (an MD step), and every result still
+    matches the oracle bit for bit (a kept buffer's old contents never show
+    through). After mnl_release_cached_memory the builds allocate afresh and
+    stay exact."""
+    import torch
+    L = _lib()
+    L.mnl_release_cached_memory()
+    rng = np.random.default_rng(6)
+    pos_a = np.ascontiguousarray(rng.uniform(0, 11, (1500, 3)))
+    pos_b = np.ascontiguousarray(pos_a + np.random.default_rng(7).normal(0, 0.02, pos_a.shape))
+    co, cell, inv, pb = _geom(11 * np.eye(3), True)
+    refs = [oracle.neighbour_list("ijdDS", co, cell, inv, pb, p, 2.0) for p in (pos_a, pos_b)]
+    Pa, Pb = (len(r["i"]) for r in refs)
+    assert Pa != Pb and all(_block_bytes(Pa * k) == _block_bytes(Pb * k) for k in (4, 8, 12, 24))
+
+    def build(which):
+        pos, ref = (pos_a, pos_b)[which], refs[which]
+        P = len(ref["i"])
+        res = ctypes.c_void_p()
+        assert L.mnl_neighbour_list_device(31, _p(co), _p(cell), _p(inv), _p(pb), 1500,
+                                           _p(pos), 0, 2.0, None, None, 0, None, -1,
+                                           ctypes.byref(res)) == 0, L.mnl_last_error()
+        got = {"i": np.empty(P, np.int32), "j": np.empty(P, np.int32),
+               "d": np.empty(P), "D": np.empty((P, 3)), "S": np.empty((P, 3), np.int32)}
+        assert L.mnl_result_npairs(res) == P
+        for q, a in got.items():
+            assert L.mnl_result_copy_to_host(res, q.encode(), _p(a)) == 0
+        assert_same_pairs(got, ref)
+        ptrs = {q: L.mnl_result_data(res, q.encode()) for q in "ijDdSc"}
+        return res, ptrs
+
+    res, first = build(0)
+    L.mnl_result_free(res)
+    res, second = build(0)
+    # i and j have the same size and may trade buffers
+    assert sorted(first.values()) == sorted(second.values())
+    assert second["D"] == first["D"] and second["S"] == first["S"]
+    L.mnl_result_free(res)
+    res, jittered = build(1)
+    assert sorted(jittered.values()) == sorted(first.values())
+
+    PyCapsule_New = ctypes.pythonapi.PyCapsule_New
+    PyCapsule_New.restype = ctypes.py_object
+    PyCapsule_New.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+    t = torch.from_dlpack(PyCapsule_New(L.mnl_result_export_dlpack(res, b"D"), b"dltensor",
+                                        None))
+    assert t.data_ptr() == jittered["D"]
+    L.mnl_result_free(res)
+    t.add_(1.0)  # the exported buffer outlives its result
+    del t
+    torch.cuda.synchronize()
+    res, third = build(0)
+    assert third["D"] == jittered["D"]
+    L.mnl_result_free(res)
+
+    L.mnl_release_cached_memory()
+    res, _ = build(1)
+    L.mnl_result_free(res)
+    L.mnl_release_cached_memory()
