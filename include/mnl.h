@@ -134,12 +134,18 @@ DLManagedTensor *mnl_result_export_dlpack(mnl_result *res, char quantity);
    highest power of two, at least 64 KiB). A freed buffer (here or by an
    exported tensor's DLPack deleter) is kept, once the device has finished
    the work queued before the free, for the next result whose buffer has its
-   class on its device; at most 16 GiB is kept in all. */
+   class on its device; at most 16 GiB is kept in all. An allocation that
+   finds the device out of memory returns that device's kept buffers first
+   and tries once more. */
 void mnl_result_free(mnl_result *res);
 
 /* Return every kept result buffer to the device (no counterpart in the
    reference, which keeps none). */
 void mnl_release_cached_memory(void);
+
+/* Bytes of result buffers kept for `device` (every device when negative);
+   no counterpart in the reference. */
+int64_t mnl_cached_bytes(int device);
 
 /* ------------------------------------------------------------------------
  * Plan API: reusable device workspace, caller-owned outputs, explicit stream.

@@ -561,10 +561,40 @@ struct CachedBlock {
     size_t bytes;
     void *ptr;
 };
-std::mutex g_blocks_mutex;
-std::vector<CachedBlock> g_blocks;
-size_t g_blocks_bytes = 0;
+/* Never destroyed: a DLPack deleter may run late in process exit. */
+struct BlockCache {
+    std::mutex mutex;
+    std::vector<CachedBlock> blocks;
+    size_t bytes = 0;
+};
+BlockCache &block_cache() {
+    static BlockCache *c = new BlockCache();
+    return *c;
+}
 constexpr size_t kBlockCacheMax = size_t(16) << 30;
+
+/* Takes the kept blocks of `device` (all devices when device < 0) out of the
+   cache and returns them to the device. */
+void release_blocks(int device) {
+    BlockCache &bc = block_cache();
+    std::vector<CachedBlock> out;
+    {
+        std::lock_guard<std::mutex> lock(bc.mutex);
+        for (size_t k = 0; k < bc.blocks.size();) {
+            if (device < 0 || bc.blocks[k].device == device) {
+                out.push_back(bc.blocks[k]);
+                bc.bytes -= bc.blocks[k].bytes;
+                bc.blocks.erase(bc.blocks.begin() + static_cast<std::ptrdiff_t>(k));
+            } else {
+                k++;
+            }
+        }
+    }
+    for (const CachedBlock &b : out) {
+        DeviceScope ds(b.device);
+        (void)hipFree(b.ptr);
+    }
+}
 
 size_t block_bytes(size_t bytes) {
     size_t top = 1;
@@ -573,20 +603,29 @@ size_t block_bytes(size_t bytes) {
     return (bytes + g - 1) / g * g;
 }
 
+/* With the device current. When the device is out of memory, the kept
+   blocks of this device go back to it and the allocation is tried once more. */
 hipError_t result_malloc(int device, void **p, size_t need) {
     const size_t bytes = block_bytes(need);
+    BlockCache &bc = block_cache();
     {
-        std::lock_guard<std::mutex> lock(g_blocks_mutex);
-        for (size_t k = 0; k < g_blocks.size(); k++) {
-            if (g_blocks[k].device == device && g_blocks[k].bytes == bytes) {
-                *p = g_blocks[k].ptr;
-                g_blocks_bytes -= bytes;
-                g_blocks.erase(g_blocks.begin() + static_cast<std::ptrdiff_t>(k));
+        std::lock_guard<std::mutex> lock(bc.mutex);
+        for (size_t k = 0; k < bc.blocks.size(); k++) {
+            if (bc.blocks[k].device == device && bc.blocks[k].bytes == bytes) {
+                *p = bc.blocks[k].ptr;
+                bc.bytes -= bytes;
+                bc.blocks.erase(bc.blocks.begin() + static_cast<std::ptrdiff_t>(k));
                 return hipSuccess;
             }
         }
     }
-    return hipMalloc(p, bytes);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        release_blocks(device);
+        e = hipMalloc(p, bytes);
+    }
+    return e;
 }
 
 /* With the device current; `need` as passed to result_malloc. */
@@ -595,21 +634,22 @@ void result_free(int device, void *p, size_t need) {
     const size_t bytes = need > 0 ? block_bytes(need) : 0;
     if (bytes > 0 && bytes <= kBlockCacheMax && hipDeviceSynchronize() == hipSuccess) {
         std::vector<void *> evict;
+        BlockCache &bc = block_cache();
         {
-            std::lock_guard<std::mutex> lock(g_blocks_mutex);
+            std::lock_guard<std::mutex> lock(bc.mutex);
             /* oldest blocks of this device make room first */
-            for (size_t k = 0; k < g_blocks.size() && g_blocks_bytes + bytes > kBlockCacheMax;) {
-                if (g_blocks[k].device == device) {
-                    evict.push_back(g_blocks[k].ptr);
-                    g_blocks_bytes -= g_blocks[k].bytes;
-                    g_blocks.erase(g_blocks.begin() + static_cast<std::ptrdiff_t>(k));
+            for (size_t k = 0; k < bc.blocks.size() && bc.bytes + bytes > kBlockCacheMax;) {
+                if (bc.blocks[k].device == device) {
+                    evict.push_back(bc.blocks[k].ptr);
+                    bc.bytes -= bc.blocks[k].bytes;
+                    bc.blocks.erase(bc.blocks.begin() + static_cast<std::ptrdiff_t>(k));
                 } else {
                     k++;
                 }
             }
-            if (g_blocks_bytes + bytes <= kBlockCacheMax) {
-                g_blocks.push_back(CachedBlock{device, bytes, p});
-                g_blocks_bytes += bytes;
+            if (bc.bytes + bytes <= kBlockCacheMax) {
+                bc.blocks.push_back(CachedBlock{device, bytes, p});
+                bc.bytes += bytes;
                 p = nullptr;
             }
         }
@@ -962,17 +1002,15 @@ DLManagedTensor *mnl_result_export_dlpack(mnl_result *r, char q) {
     return t;
 }
 
-void mnl_release_cached_memory(void) {
-    std::vector<CachedBlock> blocks;
-    {
-        std::lock_guard<std::mutex> lock(g_blocks_mutex);
-        blocks.swap(g_blocks);
-        g_blocks_bytes = 0;
-    }
-    for (const CachedBlock &b : blocks) {
-        DeviceScope ds(b.device);
-        (void)hipFree(b.ptr);
-    }
+void mnl_release_cached_memory(void) { release_blocks(-1); }
+
+int64_t mnl_cached_bytes(int device) {
+    BlockCache &bc = block_cache();
+    std::lock_guard<std::mutex> lock(bc.mutex);
+    size_t total = 0;
+    for (const CachedBlock &b : bc.blocks)
+        if (device < 0 || b.device == device) total += b.bytes;
+    return static_cast<int64_t>(total);
 }
 
 void mnl_result_free(mnl_result *r) {

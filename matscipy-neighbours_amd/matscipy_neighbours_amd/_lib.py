@@ -41,6 +41,7 @@ SIGNATURES = [
     ("mnl_result_export_dlpack", _vp, [_vp, ctypes.c_char]),
     ("mnl_result_free", None, [_vp]),
     ("mnl_release_cached_memory", None, []),
+    ("mnl_cached_bytes", ctypes.c_int64, [ctypes.c_int]),
     ("mnl_plan_create", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     ("mnl_plan_destroy", None, [_vp]),
     ("mnl_plan_device", ctypes.c_int, [_vp]),

@@ -208,3 +208,86 @@ def test_result_buffers_reused(gpu):
     res, _ = build(1)
     L.mnl_result_free(res)
     L.mnl_release_cached_memory()
+
+
+def test_kept_buffers_released_when_device_full(gpu):
+    """A one-shot build that finds the device out of memory first returns
+    the buffers the library keeps, then allocates again. 200,000 atoms of a
+    uniform gas (0.1 atoms/A^3): a build at rc 5 is freed (its ~0.55 GB of
+    outputs kept). torch then takes the device's memory in 64 MiB pieces
+    until it runs out and gives back 9 of them (0.6 GB), less than the build
+    at rc 6 needs (~0.98 GB at its size classes), so its outputs fit only
+    once the kept ones are gone, which the cache's byte count shows. The rc 6
+    list is checked on the device against D == r[j] - r[i] + S L, d == |D| <
+    rc, and its length against the count pass."""
+    import torch
+    L = _lib()
+    L.mnl_release_cached_memory()
+    n = 200_000
+    box = (n / 0.1) ** (1.0 / 3.0)
+    rng = np.random.default_rng(8)
+    r = torch.from_numpy(rng.uniform(0.0, box, (n, 3))).to("cuda:0")
+    co, cell, inv, pb = _geom(box * np.eye(3), True)
+
+    def build(rc):
+        res = ctypes.c_void_p()
+        st = L.mnl_neighbour_list_device(31, _p(co), _p(cell), _p(inv), _p(pb), n,
+                                         ctypes.c_void_p(r.data_ptr()), 1, rc, None, None, 0,
+                                         None, 0, ctypes.byref(res))
+        return st, res
+
+    st, res = build(5.0)
+    assert st == 0, L.mnl_last_error()
+    L.mnl_result_free(res)
+    cres = ctypes.c_void_p()
+    assert L.mnl_neighbour_count_device(_p(co), _p(cell), _p(inv), _p(pb), n,
+                                        ctypes.c_void_p(r.data_ptr()), 1, 6.0, None, None, 0,
+                                        None, 0, ctypes.byref(cres)) == 0
+    P6 = L.mnl_result_npairs(cres)
+    L.mnl_result_free(cres)
+    kept = L.mnl_cached_bytes(0)
+    assert kept > 0.5e9
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    free, _ = torch.cuda.mem_get_info()
+    piece = 64 << 20
+    filler = [torch.empty(free - (2 << 30), dtype=torch.uint8, device="cuda:0")]
+    try:
+        while True:
+            try:
+                filler.append(torch.empty(piece, dtype=torch.uint8, device="cuda:0"))
+            except torch.OutOfMemoryError:
+                break
+        del filler[-9:]
+        torch.cuda.empty_cache()
+        print(f"{P6} pairs at rc 6, {kept / 2**30:.2f} GiB kept, "
+              f"{len(filler) - 1} pieces held", flush=True)
+        st, res = build(6.0)
+        assert st == 0, L.mnl_last_error()
+        assert L.mnl_cached_bytes(0) == 0
+    finally:
+        del filler
+        torch.cuda.empty_cache()
+    try:
+        assert L.mnl_result_npairs(res) == P6
+        ptr = {q: L.mnl_result_data(res, q.encode()) for q in "ijDdS"}
+        i, j, D, d, S = (torch.from_dlpack(_capsule(L, res, q)) for q in "ijDdS")
+        assert i.data_ptr() == ptr["i"] and D.data_ptr() == ptr["D"]
+        ref_D = r[j.long()] - r[i.long()] + S.double() * box
+        assert float((D - ref_D).abs().max()) < 1e-9
+        assert float((d - torch.linalg.norm(D, dim=1)).abs().max()) < 1e-12
+        assert bool((d < 6.0).all()) and bool((i[1:] >= i[:-1]).all())
+        del i, j, D, d, S, ref_D
+    finally:
+        L.mnl_result_free(res)
+    torch.cuda.synchronize()
+    L.mnl_release_cached_memory()
+
+
+def _capsule(L, res, q):
+    PyCapsule_New = ctypes.pythonapi.PyCapsule_New
+    PyCapsule_New.restype = ctypes.py_object
+    PyCapsule_New.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+    t = L.mnl_result_export_dlpack(res, q.encode())
+    assert t, L.mnl_last_error()
+    return PyCapsule_New(t, b"dltensor", None)
