@@ -475,6 +475,70 @@ def leg_lj(lib, _lib, torch, dev, steps, warmup, with_cpu):
                "over the i-sorted rows; HIP events over the steps"}
 
 
+def leg_post(lib, _lib, torch, dev, steps, warmup, with_cpu):
+    """The consumers beside the list on the fcc 1M workload, through the
+    public functions with device-resident inputs and outputs: coordination
+    (count pass only, rc 5 A), neighbour_matrix (rc 5 A, 64 slots per atom),
+    first_neighbours over the rc 5 A i list (46.6 M pairs) and triplet_list
+    over the rc 3 A list (12 neighbours per atom, 132 triplets per atom).
+    HIP events on torch's stream around each call, which includes the host
+    reads of totals the functions make."""
+    import numpy as np
+
+    from matscipy_neighbours_amd import (coordination, first_neighbours, neighbour_list,
+                                         neighbour_matrix, triplet_list, workloads)
+    stream = torch.cuda.current_stream(dev)
+    w = workloads.fcc_bulk()
+    N = len(w["positions"])
+    cell = np.ascontiguousarray(w["cell"], dtype=np.float64)
+    pos = torch.from_numpy(w["positions"]).to(dev)
+    K = 64
+    geo = dict(positions=pos, cell=cell, pbc=True)
+    out = {}
+
+    def rate(ms, nbytes):
+        return {"ms": ms, "algorithmic_GB": nbytes / 1e9,
+                "achieved_GBps": nbytes / (ms * 1e-3) / 1e9,
+                "hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+    ms = _time_steps(torch, stream, lambda: coordination(cutoff=5.0, **geo), steps, warmup)
+    out["coordination_rc5"] = dict(rate(ms, 28 * N), bytes_per_atom="24 read + 4 written")
+
+    m = {}
+
+    def matrix():
+        m["r"] = neighbour_matrix(cutoff=5.0, max_neighbours=K, **geo)
+    ms = _time_steps(torch, stream, matrix, steps, warmup)
+    cnt = m["r"][2]
+    out["neighbour_matrix_rc5_K64"] = dict(
+        rate(ms, 24 * N + (4 + 24) * K * N + 4 * N), max_count=int(cnt.max()),
+        bytes_per_atom="24 read + 28 per slot (idx, D) + 4 (count) written")
+    del m["r"], cnt
+
+    i5 = neighbour_list("i", cutoff=5.0, **geo)
+    P5 = int(i5.numel())
+    ms = _time_steps(torch, stream, lambda: first_neighbours(N, i5), steps, warmup)
+    out["first_neighbours_rc5"] = dict(rate(ms, 4 * P5 + 4 * (N + 1)), pairs=P5,
+                                       bytes="4 per pair read + 4 per atom written")
+    del i5
+
+    i3 = neighbour_list("i", cutoff=3.0, **geo)
+    seed = first_neighbours(N, i3)
+    P3 = int(i3.numel())
+    t = {}
+
+    def triplets():
+        t["r"] = triplet_list(seed)
+    ms = _time_steps(torch, stream, triplets, steps, warmup)
+    T = int(t["r"][0].numel())
+    out["triplet_list_rc3"] = dict(rate(ms, 4 * (N + 1) + 8 * T), pairs=P3, triplets=T,
+                                   bytes="4 per seed read + 8 per triplet (ij, ik) written")
+    out["how"] = ("public functions on device tensors, outputs allocated by torch's caching "
+                  "allocator inside the timed call; HIP events on torch's stream, mean over "
+                  "the steps")
+    return "fcc_1M_consumers", out
+
+
 def leg_alloc(lib, _lib, torch, dev, steps, warmup, with_cpu):
     """The fcc 1M ijdDS build with the output allocation inside the timed
     region: the one-shot C entry point (a device result owning its buffers,
@@ -553,7 +617,8 @@ def leg_alloc(lib, _lib, torch, dev, steps, warmup, with_cpu):
 
 
 OTHER_LEGS = {"latency": leg_latency, "slab": leg_slab, "fcc_f32": leg_fcc_f32,
-              "autograd": leg_autograd, "lj": leg_lj, "alloc": leg_alloc}
+              "autograd": leg_autograd, "lj": leg_lj, "alloc": leg_alloc,
+              "post": leg_post}
 
 
 def other_workloads(lib, _lib, torch, dev, steps, warmup, with_cpu):

@@ -1036,10 +1036,8 @@ int matrix_enqueue(mnl_plan *p, hipStream_t st, const double origin[3], const do
                       per_type_sq, ncutoffs, types, backend, false) != MNL_SUCCESS)
         return MNL_ERROR;
     if (overflow) MNL_HIP(launch_fill_i32(overflow, 1, 0, st));
-    if (slots > 0) {
-        MNL_HIP(launch_fill_i32(idx, slots, 0, st));
-        MNL_HIP(launch_fill_i64(reinterpret_cast<int64_t *>(dist), 3 * slots, 0, st));
-    }
+    /* no clearing of idx and dist: the traversal writes every slot, the
+       unused ones (from each row's count on) as zeros */
     if (nat == 0) return MNL_SUCCESS;
     PairArgs a = pair_args(p);
     a.counts = count;

@@ -1366,11 +1366,26 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, int part, Fi
             __builtin_amdgcn_wave_barrier();
         }
         if (head > tail) flush_ring<MODE>(A, H, L, tail, head - tail);
-        /* the matrix has no count pass: the traversal's per-atom total is
-           the row count */
-        if (MODE == MODE_MATRIX && lane < mg) {
-            A.counts[H.abase + L.hi[lane]] = acc;
-            if (acc > A.max_nb && A.overflow) atomicMax(A.overflow, 1);
+        if (MODE == MODE_MATRIX) {
+            /* the matrix has no count pass: the traversal's per-atom total
+               is the row count. Each row's unused slots (from the count on)
+               are zeroed here, slots the flushes above never write, so the
+               matrix needs no clearing beforehand. */
+            if (lane < mg) {
+                A.counts[H.abase + L.hi[lane]] = acc;
+                if (acc > A.max_nb && A.overflow) atomicMax(A.overflow, 1);
+            }
+            double *dist = static_cast<double *>(A.out_D);
+            for (int t = 0; t < mg; t++) {
+                const int used = min(__builtin_amdgcn_readlane(acc, t), A.max_nb);
+                const int64_t row = L.hpos[t];
+                for (int sl = used + lane; sl < A.max_nb; sl += kWave) {
+                    A.out_j[row + sl] = 0;
+                    dist[3 * (row + sl)] = 0.0;
+                    dist[3 * (row + sl) + 1] = 0.0;
+                    dist[3 * (row + sl) + 2] = 0.0;
+                }
+            }
         }
     }
 }

@@ -63,7 +63,10 @@ def test_matrix_matches_oracle(gpu, seed):
     inv = np.ascontiguousarray(np.linalg.inv(cell.T))
     rc = float(rng.uniform(1.5, 4.0))
     ref = oracle.neighbour_matrix(np.zeros(3), cell, inv, pbc, pos, rc, 1)
-    for K in (max(1, int(ref[2].max()) - 3), int(ref[2].max()), int(ref[2].max()) + 5):
+    # K below the largest row (overflow), at it, and above it by a few and
+    # by more than 64 slots (rows whose zeroed tail takes several passes)
+    for K in (max(1, int(ref[2].max()) - 3), int(ref[2].max()), int(ref[2].max()) + 5,
+              int(ref[2].max()) + 70):
         want = oracle.neighbour_matrix(np.zeros(3), cell, inv, pbc, pos, rc, K)
         got = _engine_matrix(pos, cell, inv, pbc, rc, None, K)
         assert np.array_equal(got[0], want[0]) and np.array_equal(got[2], want[2])
