@@ -150,6 +150,7 @@ struct mnl_plan {
     int32_t tri_rows = 0;
     int64_t tri_total = -1;
     const int32_t *tri_first = nullptr;
+    bool tri_all = false; /* no distance cutoff: every pair kept, no index table */
     /* batched geometry uploaded by mnl_plan_set_batch and still in place */
     bool batch_ready = false;
 };
@@ -1508,6 +1509,7 @@ int mnl_plan_triplets_count(mnl_plan *p, void *stream, int32_t n_first, const in
     p->tri_total = *p->host_total;
     p->tri_rows = rows;
     p->tri_first = first_i;
+    p->tri_all = absdist == nullptr;
     *ntriplets = p->tri_total;
     return MNL_SUCCESS;
 }
@@ -1517,8 +1519,10 @@ int mnl_plan_triplets_fill(mnl_plan *p, void *stream, int32_t *ij, int32_t *ik) 
     if (p->tri_total < 0) return fail("mnl_plan_triplets_fill without mnl_plan_triplets_count");
     if (p->tri_total > 0 && (!ij || !ik)) return fail("triplet output is NULL");
     DeviceScope ds(p->device);
+    /* without a distance cutoff every pair is kept: no index table */
     MNL_HIP(launch_triplet_fill(p->tri_first, p->tri_rows, p->tri_m.as<int32_t>(),
-                                p->tri_off.as<int64_t>(), p->tri_vidx.as<int32_t>(), ij, ik,
+                                p->tri_off.as<int64_t>(),
+                                p->tri_all ? nullptr : p->tri_vidx.as<int32_t>(), ij, ik,
                                 static_cast<hipStream_t>(stream)));
     return MNL_SUCCESS;
 }

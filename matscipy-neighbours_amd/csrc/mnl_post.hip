@@ -100,9 +100,30 @@ __global__ __launch_bounds__(kBlock) void k_triplet_rows(const int32_t *__restri
     }
 }
 
-/* One wave per row: triplet t of the row is (kept[a], kept[c]) with
-   a = t / (m - 1) and c the (t mod (m - 1))-th kept index other than a:
-   ij outer, ik inner, both ascending - the reference's loop order. */
+/* Rows without a distance cutoff keep every pair: one thread per row,
+   m = the row's length, tcnt = m (m - 1); the kept indices are the pair
+   indices themselves (no compaction). */
+__global__ __launch_bounds__(kBlock) void k_triplet_rows_all(const int32_t *__restrict__ first_i,
+                                                             int32_t nrows, int64_t nn,
+                                                             int32_t *__restrict__ m,
+                                                             int64_t *__restrict__ tcnt) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (r >= nrows) return;
+    const int32_t b = first_i[r];
+    const int64_t e = first_i[r + 1] < nn ? first_i[r + 1] : nn;
+    const int cnt = b >= 0 && e > b ? static_cast<int>(e - b) : 0;
+    m[r] = cnt;
+    tcnt[r] = static_cast<int64_t>(cnt) * (cnt > 0 ? cnt - 1 : 0);
+}
+
+/* One wave per 64 consecutive rows (their m, first index and triplet
+   offset arrive in one coalesced load each). Triplet t of a row is
+   (kept[a], kept[c]) with a = t / (m - 1) and c the (t mod (m - 1))-th kept
+   index other than a: ij outer, ik inner, both ascending - the reference's
+   loop order. Lanes take t = lane, lane + 64, ...; (a, t mod (m - 1)) steps
+   by (64 / (m - 1), 64 mod (m - 1)) with one carry, so there is one division
+   per row, not per triplet. kept[x] is vidx[b + x], or b + x when vidx is
+   NULL (every pair kept). */
 __global__ __launch_bounds__(kBlock) void k_triplet_fill(const int32_t *__restrict__ first_i,
                                                          int32_t nrows,
                                                          const int32_t *__restrict__ m,
@@ -110,20 +131,44 @@ __global__ __launch_bounds__(kBlock) void k_triplet_fill(const int32_t *__restri
                                                          const int32_t *__restrict__ vidx,
                                                          int32_t *__restrict__ ij,
                                                          int32_t *__restrict__ ik) {
-    const int r = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-    if (r >= nrows) return;
+    const int64_t r0 =
+        (static_cast<int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave) * kWave;
+    if (r0 >= nrows) return;
     const int lane = __lane_id();
-    const int mr = m[r];
-    if (mr < 2) return;
-    const int32_t b = first_i[r];
-    const int64_t base = toff[r];
-    const int64_t total = static_cast<int64_t>(mr) * (mr - 1);
-    for (int64_t t = lane; t < total; t += kWave) {
-        const int a = static_cast<int>(t / (mr - 1));
-        const int c0 = static_cast<int>(t - static_cast<int64_t>(a) * (mr - 1));
-        const int c = c0 + (c0 >= a ? 1 : 0);
-        ij[base + t] = vidx[b + a];
-        ik[base + t] = vidx[b + c];
+    const int64_t rl = r0 + lane;
+    int m_l = 0, b_l = 0;
+    int64_t base_l = 0;
+    if (rl < nrows) {
+        m_l = m[rl];
+        b_l = first_i[rl];
+        base_l = toff[rl];
+    }
+    const int nr = static_cast<int>(nrows - r0 < kWave ? nrows - r0 : kWave);
+    for (int q = 0; q < nr; q++) {
+        const int mr = __builtin_amdgcn_readlane(m_l, q);
+        if (mr < 2) continue;
+        const int b = __builtin_amdgcn_readlane(b_l, q);
+        const int64_t base =
+            static_cast<int64_t>(static_cast<uint32_t>(
+                __builtin_amdgcn_readlane(static_cast<int>(base_l), q))) |
+            (static_cast<int64_t>(__builtin_amdgcn_readlane(static_cast<int>(base_l >> 32), q))
+             << 32);
+        const int d = mr - 1;
+        const int64_t total = static_cast<int64_t>(mr) * d;
+        const int qa = kWave / d, rem = kWave - qa * d;
+        int a = lane / d;
+        int c0 = lane - a * d;
+        for (int64_t t = lane; t < total; t += kWave) {
+            const int c = c0 + (c0 >= a ? 1 : 0);
+            ij[base + t] = vidx ? vidx[b + a] : b + a;
+            ik[base + t] = vidx ? vidx[b + c] : b + c;
+            a += qa;
+            c0 += rem;
+            if (c0 >= d) {
+                c0 -= d;
+                a++;
+            }
+        }
     }
 }
 
@@ -148,6 +193,11 @@ hipError_t launch_triplet_rows(const int32_t *first_i, int32_t nrows, int64_t nn
                                const double *absdist, double cutoff, int32_t *m, int64_t *tcnt,
                                int32_t *vidx, hipStream_t stream) {
     if (nrows <= 0) return hipSuccess;
+    if (!absdist) {
+        k_triplet_rows_all<<<blocks_for(nrows), kBlock, 0, stream>>>(first_i, nrows, nn, m,
+                                                                     tcnt);
+        return hipGetLastError();
+    }
     const unsigned nb = static_cast<unsigned>((nrows + kBlock / kWave - 1) / (kBlock / kWave));
     k_triplet_rows<<<nb, kBlock, 0, stream>>>(first_i, nrows, nn, absdist, cutoff, m, tcnt,
                                               vidx);
@@ -158,7 +208,8 @@ hipError_t launch_triplet_fill(const int32_t *first_i, int32_t nrows, const int3
                                const int64_t *toff, const int32_t *vidx, int32_t *ij,
                                int32_t *ik, hipStream_t stream) {
     if (nrows <= 0) return hipSuccess;
-    const unsigned nb = static_cast<unsigned>((nrows + kBlock / kWave - 1) / (kBlock / kWave));
+    const int64_t rows_per_block = int64_t(kBlock / kWave) * kWave;
+    const unsigned nb = static_cast<unsigned>((nrows + rows_per_block - 1) / rows_per_block);
     k_triplet_fill<<<nb, kBlock, 0, stream>>>(first_i, nrows, m, toff, vidx, ij, ik);
     return hipGetLastError();
 }

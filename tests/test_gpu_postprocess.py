@@ -70,3 +70,31 @@ def test_empty_and_edge_cases(gpu):
     ij, ik = triplet_list(np.array([0, 3], np.int32), d, 0.9)
     ref = oracle.triplet_list(np.array([0, 3], np.int32), d, 0.9)
     assert ij.tolist() == ref[0].tolist() and ik.tolist() == ref[1].tolist()
+
+
+@pytest.mark.parametrize("with_cutoff", [False, True])
+def test_triplet_row_lengths(gpu, with_cutoff):
+    """Rows of 0 to 200 pairs, whose triplet indices step by 64 / (m - 1)
+    and 64 mod (m - 1) per lane pass (m - 1 below, at and above 64), over
+    331 rows (64 rows per wave, the last wave partial), with and without a
+    distance cutoff, against the oracle; host and device input."""
+    import torch
+
+    from matscipy_neighbours_amd import triplet_list
+    rng = np.random.default_rng(31)
+    lengths = np.concatenate([[0, 1, 2, 3, 63, 64, 65, 66, 127, 128, 129, 200, 0, 5],
+                              rng.integers(0, 40, 317)])
+    seed = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int32)
+    args = (seed,)
+    if with_cutoff:
+        d = rng.uniform(0.0, 2.0, int(seed[-1]))
+        args = (seed, d, 1.3)
+    want = oracle.triplet_list(*args)
+    got = triplet_list(*args)
+    assert len(want[0]) > 100_000
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    dev = tuple(torch.from_numpy(np.asarray(a)).to("cuda:0") if isinstance(a, np.ndarray)
+                else a for a in args)
+    ij, ik = triplet_list(*dev)
+    assert np.array_equal(ij.cpu().numpy(), want[0]) and np.array_equal(ik.cpu().numpy(),
+                                                                        want[1])
