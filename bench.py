@@ -480,7 +480,8 @@ def leg_post(lib, _lib, torch, dev, steps, warmup, with_cpu):
     public functions with device-resident inputs and outputs: coordination
     (count pass only, rc 5 A), neighbour_matrix (rc 5 A, 64 slots per atom),
     first_neighbours over the rc 5 A i list (46.6 M pairs) and triplet_list
-    over the rc 3 A list (12 neighbours per atom, 132 triplets per atom).
+    over the rc 3 A list (12 neighbours per atom, 132 triplets per atom),
+    and with the distance cutoff 2.55 A (about half the pairs kept).
     HIP events on torch's stream around each call, which includes the host
     reads of totals the functions make."""
     import numpy as np
@@ -533,6 +534,16 @@ def leg_post(lib, _lib, torch, dev, steps, warmup, with_cpu):
     T = int(t["r"][0].numel())
     out["triplet_list_rc3"] = dict(rate(ms, 4 * (N + 1) + 8 * T), pairs=P3, triplets=T,
                                    bytes="4 per seed read + 8 per triplet (ij, ik) written")
+    del t["r"]
+    d3 = neighbour_list("d", cutoff=3.0, **geo)
+
+    def triplets_cut():
+        t["r"] = triplet_list(seed, d3, 2.55)
+    ms = _time_steps(torch, stream, triplets_cut, steps, warmup)
+    Tc = int(t["r"][0].numel())
+    out["triplet_list_rc3_cutoff2.55"] = dict(
+        rate(ms, 4 * (N + 1) + 8 * P3 + 8 * Tc), pairs=P3, triplets=Tc,
+        bytes="4 per seed + 8 per pair (d) read + 8 per triplet written")
     out["how"] = ("public functions on device tensors, outputs allocated by torch's caching "
                   "allocator inside the timed call; HIP events on torch's stream, mean over "
                   "the steps")

@@ -126,7 +126,8 @@ struct mnl_plan {
     DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, scount, slot,
         counts, order_stats,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
-        htypes, cell_partial, tri_m, tri_cnt, tri_off, tri_vidx, tri_flag, lj, tune_out;
+        htypes, cell_partial, tri_m, tri_cnt, tri_off, tri_vidx, tri_vstart, tri_flag, lj,
+        tune_out;
     int64_t *host_total = nullptr; /* pinned */
     /* state of the last count */
     bool counted = false;
@@ -823,7 +824,7 @@ void mnl_plan_destroy(mnl_plan *p) {
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
                       &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
                       &p->cell_partial, &p->tri_m,   &p->tri_cnt,  &p->tri_off,
-                      &p->tri_vidx,  &p->tri_flag,  &p->lj,     &p->tune_out};
+                      &p->tri_vidx,  &p->tri_vstart, &p->tri_flag, &p->lj, &p->tune_out};
     for (DevBuf *b : bufs) b->release();
     if (p->host_total) (void)hipHostFree(p->host_total);
     delete p;
@@ -1491,6 +1492,7 @@ int mnl_plan_triplets_count(mnl_plan *p, void *stream, int32_t n_first, const in
                  {&p->tri_cnt, r1 * sizeof(int64_t)},
                  {&p->tri_off, (r1 + 1) * sizeof(int64_t)},
                  {&p->tri_vidx, std::max<size_t>(1, static_cast<size_t>(nn)) * sizeof(int32_t)},
+                 {&p->tri_vstart, r1 * sizeof(int32_t)},
                  {&p->tri_flag, sizeof(int64_t)},
                  {&p->partial, static_cast<size_t>(scan_partial_size(rows)) * sizeof(int64_t)}};
     bool grow = false;
@@ -1500,7 +1502,8 @@ int mnl_plan_triplets_count(mnl_plan *p, void *stream, int32_t n_first, const in
         for (auto &nd : needs) MNL_HIP(nd.b->ensure(nd.bytes));
     }
     MNL_HIP(launch_triplet_rows(first_i, rows, nn, absdist, cutoff, p->tri_m.as<int32_t>(),
-                                p->tri_cnt.as<int64_t>(), p->tri_vidx.as<int32_t>(), st));
+                                p->tri_cnt.as<int64_t>(), p->tri_vstart.as<int32_t>(),
+                                p->tri_vidx.as<int32_t>(), st));
     MNL_HIP(launch_scan_i64_to_i64(p->tri_cnt.as<int64_t>(), rows, p->tri_off.as<int64_t>(),
                                    p->partial.as<int64_t>(), p->tri_flag.as<int64_t>(), st));
     MNL_HIP(hipMemcpyAsync(p->host_total, p->tri_flag.ptr, sizeof(int64_t),
@@ -1520,7 +1523,8 @@ int mnl_plan_triplets_fill(mnl_plan *p, void *stream, int32_t *ij, int32_t *ik) 
     if (p->tri_total > 0 && (!ij || !ik)) return fail("triplet output is NULL");
     DeviceScope ds(p->device);
     /* without a distance cutoff every pair is kept: no index table */
-    MNL_HIP(launch_triplet_fill(p->tri_first, p->tri_rows, p->tri_m.as<int32_t>(),
+    MNL_HIP(launch_triplet_fill(p->tri_all ? p->tri_first : p->tri_vstart.as<int32_t>(),
+                                p->tri_rows, p->tri_m.as<int32_t>(),
                                 p->tri_off.as<int64_t>(),
                                 p->tri_all ? nullptr : p->tri_vidx.as<int32_t>(), ij, ik,
                                 static_cast<hipStream_t>(stream)));

@@ -152,12 +152,15 @@ hipError_t launch_first_neighbours(const int32_t *i, int64_t nn, int32_t n, int3
                                    hipStream_t stream);
 hipError_t launch_count_jumps(const int32_t *sorted, int64_t nn, int32_t *count,
                               hipStream_t stream);
-/* rows r < nrows (= n_first - 1): m[r] kept pairs, tcnt[r] = m (m - 1),
-   vidx[first_i[r] + k] = k-th kept pair of the row (row ends clamped to nn) */
+/* rows r < nrows (= n_first - 1): m[r] kept pairs, tcnt[r] = m (m - 1)
+   (row ends clamped to nn). With absdist: vidx[vstart[r] + k] = k-th kept
+   pair of the row; without, every pair is kept and vstart, vidx are not
+   written. */
 hipError_t launch_triplet_rows(const int32_t *first_i, int32_t nrows, int64_t nn,
                                const double *absdist, double cutoff, int32_t *m, int64_t *tcnt,
-                               int32_t *vidx, hipStream_t stream);
-hipError_t launch_triplet_fill(const int32_t *first_i, int32_t nrows, const int32_t *m,
+                               int32_t *vstart, int32_t *vidx, hipStream_t stream);
+/* vbase: vstart (with vidx) or first_i (vidx NULL: every pair kept) */
+hipError_t launch_triplet_fill(const int32_t *vbase, int32_t nrows, const int32_t *m,
                                const int64_t *toff, const int32_t *vidx, int32_t *ij,
                                int32_t *ik, hipStream_t stream);
 /* Pair kernels (one wave per home cell). mode 0 = count, 1 = fill f64

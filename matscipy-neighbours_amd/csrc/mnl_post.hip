@@ -66,37 +66,54 @@ __global__ __launch_bounds__(kBlock) void k_count_jumps(const int32_t *__restric
     }
 }
 
-/* One wave per row: compact the row's kept pairs (!(d >= cutoff), so NaN
-   distances are kept, as in the reference's test) into vidx in ascending
-   order; m = kept count, tcnt = m (m - 1) triplets. A row starting at a
-   negative index (rows before the first atom with pairs) is empty. */
+/* With a distance cutoff: one wave per 64 consecutive rows. Their pairs
+   form one contiguous range [lo, hi), which the wave compacts chunk by
+   chunk (kept: !(d >= cutoff), so NaN distances are kept, as in the
+   reference's test) into vidx[lo ...] in ascending order. A row's kept pairs
+   then start at vstart = lo + (kept pairs before the row's first pair),
+   read off the ballot of the chunk holding that first pair; m = the next
+   row's start minus its own, tcnt = m (m - 1). Rows whose first index is
+   negative (before the first atom with pairs) are empty. */
 __global__ __launch_bounds__(kBlock) void k_triplet_rows(const int32_t *__restrict__ first_i,
                                                          int32_t nrows, int64_t nn,
                                                          const double *__restrict__ absdist,
                                                          double cutoff, int32_t *__restrict__ m,
                                                          int64_t *__restrict__ tcnt,
+                                                         int32_t *__restrict__ vstart,
                                                          int32_t *__restrict__ vidx) {
-    const int r = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-    if (r >= nrows) return;
+    const int64_t r0 =
+        (static_cast<int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave) * kWave;
+    if (r0 >= nrows) return;
     const int lane = __lane_id();
-    const int32_t b = first_i[r];
-    const int32_t e = static_cast<int32_t>(first_i[r + 1] < nn ? first_i[r + 1] : nn);
-    int cnt = 0;
-    if (b >= 0) {
-        for (int32_t k0 = b; k0 < e; k0 += kWave) {
-            const int32_t k = k0 + lane;
-            const bool keep = k < e && !(absdist && absdist[k] >= cutoff);
-            const uint64_t mask = __builtin_amdgcn_ballot_w64(keep);
-            const int pre = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-                                                      __builtin_amdgcn_mbcnt_lo(
-                                                          static_cast<uint32_t>(mask), 0u));
-            if (keep) vidx[b + cnt + pre] = k;
-            cnt += __popcll(mask);
-        }
+    const int nr = static_cast<int>(nrows - r0 < kWave ? nrows - r0 : kWave);
+    const int32_t b = lane < nr ? first_i[r0 + lane] : -1;
+    const int64_t end_first = first_i[r0 + nr];
+    const int64_t hi = end_first < nn ? end_first : nn;
+    int32_t lo = b >= 0 ? b : INT32_MAX;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) lo = min(lo, __shfl_xor(lo, o, kWave));
+    int gp = -1; /* kept pairs in [lo, b) */
+    int run = 0;
+    for (int64_t k0 = lo; k0 < hi; k0 += kWave) {
+        const int64_t k = k0 + lane;
+        const bool keep = k < hi && !(absdist[k] >= cutoff);
+        const uint64_t mask = __builtin_amdgcn_ballot_w64(keep);
+        const int pre = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo(
+                                                      static_cast<uint32_t>(mask), 0u));
+        if (keep) vidx[lo + run + pre] = static_cast<int32_t>(k);
+        if (b >= k0 && b < k0 + kWave)
+            gp = run + static_cast<int>(__popcll(mask & ((1ull << (b - k0)) - 1)));
+        run += static_cast<int>(__popcll(mask));
     }
-    if (lane == 0) {
-        m[r] = cnt;
-        tcnt[r] = static_cast<int64_t>(cnt) * (cnt > 0 ? cnt - 1 : 0);
+    if (b >= 0 && gp < 0) gp = run; /* first pair at or past hi: an empty row */
+    const int gnext_l = __shfl_down(gp, 1, kWave);
+    const int gnext = lane == nr - 1 ? run : gnext_l;
+    if (lane < nr) {
+        const int cnt = b >= 0 ? gnext - gp : 0;
+        m[r0 + lane] = cnt;
+        tcnt[r0 + lane] = static_cast<int64_t>(cnt) * (cnt > 0 ? cnt - 1 : 0);
+        vstart[r0 + lane] = b >= 0 ? lo + gp : 0;
     }
 }
 
@@ -122,9 +139,10 @@ __global__ __launch_bounds__(kBlock) void k_triplet_rows_all(const int32_t *__re
    index other than a: ij outer, ik inner, both ascending - the reference's
    loop order. Lanes take t = lane, lane + 64, ...; (a, t mod (m - 1)) steps
    by (64 / (m - 1), 64 mod (m - 1)) with one carry, so there is one division
-   per row, not per triplet. kept[x] is vidx[b + x], or b + x when vidx is
-   NULL (every pair kept). */
-__global__ __launch_bounds__(kBlock) void k_triplet_fill(const int32_t *__restrict__ first_i,
+   per row, not per triplet. kept[x] is vidx[b + x] with b the row's start
+   in the compacted table, or b + x with b its first pair when vidx is NULL
+   (every pair kept). */
+__global__ __launch_bounds__(kBlock) void k_triplet_fill(const int32_t *__restrict__ vbase,
                                                          int32_t nrows,
                                                          const int32_t *__restrict__ m,
                                                          const int64_t *__restrict__ toff,
@@ -140,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_triplet_fill(const int32_t *__restri
     int64_t base_l = 0;
     if (rl < nrows) {
         m_l = m[rl];
-        b_l = first_i[rl];
+        b_l = vbase[rl];
         base_l = toff[rl];
     }
     const int nr = static_cast<int>(nrows - r0 < kWave ? nrows - r0 : kWave);
@@ -191,26 +209,27 @@ hipError_t launch_count_jumps(const int32_t *sorted, int64_t nn, int32_t *count,
 
 hipError_t launch_triplet_rows(const int32_t *first_i, int32_t nrows, int64_t nn,
                                const double *absdist, double cutoff, int32_t *m, int64_t *tcnt,
-                               int32_t *vidx, hipStream_t stream) {
+                               int32_t *vstart, int32_t *vidx, hipStream_t stream) {
     if (nrows <= 0) return hipSuccess;
     if (!absdist) {
         k_triplet_rows_all<<<blocks_for(nrows), kBlock, 0, stream>>>(first_i, nrows, nn, m,
                                                                      tcnt);
         return hipGetLastError();
     }
-    const unsigned nb = static_cast<unsigned>((nrows + kBlock / kWave - 1) / (kBlock / kWave));
+    const int64_t rows_per_block = int64_t(kBlock / kWave) * kWave;
+    const unsigned nb = static_cast<unsigned>((nrows + rows_per_block - 1) / rows_per_block);
     k_triplet_rows<<<nb, kBlock, 0, stream>>>(first_i, nrows, nn, absdist, cutoff, m, tcnt,
-                                              vidx);
+                                              vstart, vidx);
     return hipGetLastError();
 }
 
-hipError_t launch_triplet_fill(const int32_t *first_i, int32_t nrows, const int32_t *m,
+hipError_t launch_triplet_fill(const int32_t *vbase, int32_t nrows, const int32_t *m,
                                const int64_t *toff, const int32_t *vidx, int32_t *ij,
                                int32_t *ik, hipStream_t stream) {
     if (nrows <= 0) return hipSuccess;
     const int64_t rows_per_block = int64_t(kBlock / kWave) * kWave;
     const unsigned nb = static_cast<unsigned>((nrows + rows_per_block - 1) / rows_per_block);
-    k_triplet_fill<<<nb, kBlock, 0, stream>>>(first_i, nrows, m, toff, vidx, ij, ik);
+    k_triplet_fill<<<nb, kBlock, 0, stream>>>(vbase, nrows, m, toff, vidx, ij, ik);
     return hipGetLastError();
 }
 

@@ -70,6 +70,14 @@ def test_empty_and_edge_cases(gpu):
     ij, ik = triplet_list(np.array([0, 3], np.int32), d, 0.9)
     ref = oracle.triplet_list(np.array([0, 3], np.int32), d, 0.9)
     assert ij.tolist() == ref[0].tolist() and ik.tolist() == ref[1].tolist()
+    # rows before the first atom with pairs (-1), empty rows in the middle
+    # and at the end, with and without a cutoff
+    seed = np.array([-1, -1, 0, 3, 3, 5, 5], np.int32)
+    d = np.array([0.1, 0.2, 2.0, 0.3, 0.4])
+    for args in ((seed,), (seed, d, 1.0)):
+        got, ref = triplet_list(*args), oracle.triplet_list(*args)
+        assert got[0].tolist() == ref[0].tolist() and got[1].tolist() == ref[1].tolist()
+    assert oracle.triplet_list(seed, d, 1.0)[0].tolist() == [0, 1, 3, 4]
 
 
 @pytest.mark.parametrize("with_cutoff", [False, True])
