@@ -260,11 +260,13 @@ class _Build:
         self.pos = positions_t
         self.backend = backend
 
-    def run(self, flags, want_counts=False, f32=False, packed=False):
+    def run(self, flags, want_counts=False, f32=False, packed=False, need_total=True):
         """Count, then fill the quantities in ``flags``. With ``packed`` the
         outputs are views of one device buffer (``out["_packed"]``: D and d
         first, then i, j, S, each part aligned to its element size), so that a
-        host result needs a single transfer."""
+        host result needs a single transfer. ``need_total=False`` (no
+        quantities) leaves the pair total unread, so the call does not wait
+        for the device."""
         torch = self.torch
         lib = _lib.load()
         idx = self.dev.index if self.dev.index is not None else torch.cuda.current_device()
@@ -273,13 +275,17 @@ class _Build:
         with plan.lock, torch.cuda.device(idx):
             stream = torch.cuda.current_stream(idx).cuda_stream
             npairs = _lib.ctypes.c_int64(0)
-            _lib.check(lib.mnl_plan_count(
-                plan.handle, _lib.ctypes.c_void_p(stream), _lib.ptr(self.co),
-                _lib.ptr(self.ce), _lib.ptr(self.inv), _lib.ptr(self.pbc), self.nat,
-                _lib.ptr(self.pos), self.rc, _lib.ptr(self.per_atom_t),
-                _lib.ptr(self.per_type_sq), self.nc, _lib.ptr(self.types_t),
-                int(self.backend), _lib.ctypes.byref(npairs)))
-            P = int(npairs.value)
+            args = (plan.handle, _lib.ctypes.c_void_p(stream), _lib.ptr(self.co),
+                    _lib.ptr(self.ce), _lib.ptr(self.inv), _lib.ptr(self.pbc), self.nat,
+                    _lib.ptr(self.pos), self.rc, _lib.ptr(self.per_atom_t),
+                    _lib.ptr(self.per_type_sq), self.nc, _lib.ptr(self.types_t),
+                    int(self.backend))
+            if flags or need_total:
+                _lib.check(lib.mnl_plan_count(*args, _lib.ctypes.byref(npairs)))
+                P = int(npairs.value)
+            else:
+                _lib.check(lib.mnl_plan_count_async(*args, None))
+                P = None
             fdt = torch.float32 if f32 else torch.float64
             kw = dict(device=self.dev)
             parts = [(q, shape, dt) for q, bit, shape, dt in (
@@ -574,7 +580,7 @@ def coordination(atoms=None, cutoff=None, *, positions=None, cell=None, pbc=None
         atoms, positions, cell, pbc, numbers, cell_origin, device)
     rc, nums = _resolve_cutoff(cutoff, nums)
     b = _Build(pos_t, co, ce, inv, pb, rc, nums)
-    out = b.run(0, want_counts=True)
+    out = b.run(0, want_counts=True, need_total=False)
     counts = out["c"]
     if not want_dev and array_namespace is None:
         return counts.cpu().numpy().astype(np.int64)

@@ -146,3 +146,27 @@ def test_atoms_and_explicit_arrays_are_exclusive(gpu):
     a = _formic_acid()
     with pytest.raises(ValueError):
         neighbour_list("i", a, 1.85, positions=a.positions)
+
+
+def test_coordination_matches_oracle_back_to_back(gpu):
+    """coordination leaves the pair total unread (nothing waits for the
+    device): two calls in a row on device positions, then the host copies of
+    both results, each equal to the oracle's per-atom pair counts; and the
+    host-array form."""
+    import torch
+
+    import oracle
+    from matscipy_neighbours_amd import coordination
+    rng = np.random.default_rng(44)
+    cell = np.array([[11.0, 0.0, 0.0], [1.5, 12.0, 0.0], [-0.7, 0.9, 10.5]])
+    inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+    pos = [rng.uniform(-0.1, 1.1, (1800, 3)) @ cell for _ in range(2)]
+    got = [coordination(positions=torch.from_numpy(p).to("cuda:0"), cell=cell,
+                        pbc=(True, True, False), cutoff=2.7) for p in pos]
+    for p, g in zip(pos, got):
+        ref = oracle.neighbour_list("i", np.zeros(3), cell, inv, np.array([True, True, False]),
+                                    p, 2.7)["i"]
+        want = np.bincount(ref, minlength=len(p))
+        assert np.array_equal(g.cpu().numpy(), want)
+        assert np.array_equal(coordination(positions=p, cell=cell, pbc=(True, True, False),
+                                           cutoff=2.7), want)
