@@ -29,22 +29,41 @@ __global__ void k_seed_fill(int32_t *seed, int64_t n, int32_t v) {
 
 /* seed[l] = first pair of row l: the pair at k starts rows i[k-1]+1 .. i[k]
    when i changes at k (rows without pairs point at the next row's start),
-   rows before i[0] keep -1, rows after the last get nn. */
-__global__ void k_first_neighbours(const int32_t *__restrict__ i, int64_t nn, int32_t n,
-                                   int32_t *__restrict__ seed) {
-    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= nn) return;
-    /* indices outside [0, n] are ignored (never written out of bounds) */
-    const int32_t ik = i[k];
+   rows before i[0] keep -1, rows after the last get nn. Indices outside
+   [0, n] are ignored (never written out of bounds). ik = i[k], ip = i[k-1]
+   (unused for k = 0). */
+__device__ __forceinline__ void seed_at(int64_t k, int32_t ik, int32_t ip, int64_t nn,
+                                        int32_t n, int32_t *__restrict__ seed) {
     if (k == 0) {
         if (ik >= 0 && ik <= n) seed[ik] = 0;
     } else {
-        const int32_t ip = i[k - 1];
         const int32_t hi = ik < n ? ik : n;
         for (int32_t l = (ip + 1 > 0 ? ip + 1 : 0); l <= hi; l++) seed[l] = static_cast<int32_t>(k);
     }
     if (k == nn - 1)
         for (int32_t l = (ik + 1 > 0 ? ik + 1 : 0); l <= n; l++) seed[l] = static_cast<int32_t>(nn);
+}
+
+/* Pairs k0 .. nn-1, one per thread. */
+__global__ void k_first_neighbours(const int32_t *__restrict__ i, int64_t k0, int64_t nn,
+                                   int32_t n, int32_t *__restrict__ seed) {
+    const int64_t k = k0 + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    seed_at(k, i[k], k > 0 ? i[k - 1] : 0, nn, n, seed);
+}
+
+/* Pairs 0 .. 4 nq - 1, four per thread from one 16-byte load (i 16-byte
+   aligned). */
+__global__ void k_first_neighbours4(const int32_t *__restrict__ i, int64_t nq, int64_t nn,
+                                    int32_t n, int32_t *__restrict__ seed) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int4 v = reinterpret_cast<const int4 *>(i)[q];
+    const int64_t k = 4 * q;
+    seed_at(k, v.x, k > 0 ? i[k - 1] : 0, nn, n, seed);
+    seed_at(k + 1, v.y, v.x, nn, n, seed);
+    seed_at(k + 2, v.z, v.y, nn, n, seed);
+    seed_at(k + 3, v.w, v.z, nn, n, seed);
 }
 
 /* Number of positions where a sorted array changes value (block sums, one
@@ -196,7 +215,14 @@ hipError_t launch_first_neighbours(const int32_t *i, int64_t nn, int32_t n, int3
                                    hipStream_t stream) {
     k_seed_fill<<<blocks_for(int64_t(n) + 1), kBlock, 0, stream>>>(seed, int64_t(n) + 1,
                                                                    nn > 0 ? -1 : 0);
-    if (nn > 0) k_first_neighbours<<<blocks_for(nn), kBlock, 0, stream>>>(i, nn, n, seed);
+    if (nn <= 0) return hipGetLastError();
+    const bool aligned = (reinterpret_cast<uintptr_t>(i) & 15) == 0;
+    const int64_t nq = aligned ? nn / 4 : 0;
+    if (nq > 0)
+        k_first_neighbours4<<<blocks_for(nq), kBlock, 0, stream>>>(i, nq, nn, n, seed);
+    if (4 * nq < nn)
+        k_first_neighbours<<<blocks_for(nn - 4 * nq), kBlock, 0, stream>>>(i, 4 * nq, nn, n,
+                                                                            seed);
     return hipGetLastError();
 }
 

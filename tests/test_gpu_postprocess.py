@@ -106,3 +106,21 @@ def test_triplet_row_lengths(gpu, with_cutoff):
     ij, ik = triplet_list(*dev)
     assert np.array_equal(ij.cpu().numpy(), want[0]) and np.array_equal(ik.cpu().numpy(),
                                                                         want[1])
+
+
+def test_first_neighbours_alignment_and_tails(gpu):
+    """first_neighbours reads four pairs per thread where the list is 16-byte
+    aligned and one per thread for the rest: device slices starting 0 to 3
+    elements into the list (aligned and not) and lengths that leave 0 to 3
+    pairs over, against the oracle."""
+    import torch
+
+    from matscipy_neighbours_amd import first_neighbours
+    rng = np.random.default_rng(17)
+    i = np.sort(rng.integers(3, 5000, 40_003)).astype(np.int32)
+    it = torch.from_numpy(i).to("cuda:0")
+    for start in range(4):
+        for stop in (len(i), len(i) - 1, len(i) - 2, len(i) - 3, start + 5, start + 1):
+            got = first_neighbours(5003, it[start:stop])
+            want = oracle.first_neighbours(5003, i[start:stop])
+            assert np.array_equal(got.cpu().numpy(), want), (start, stop)
