@@ -1922,7 +1922,7 @@ static void launch_count_t(const PairArgs &a, unsigned nblocks, hipStream_t stre
    and a second sweep over the home atoms for the rest); 4 otherwise. */
 template <int MODE, int CUT, bool HASHED>
 static void launch_fill_t(const PairArgs &a, unsigned nblocks, hipStream_t stream) {
-    if (a.fill_chunks == 5 && MODE != MODE_MATRIX)
+    if (a.fill_chunks == 5)
         k_fill<MODE, CUT, HASHED, 5><<<nblocks, kBlock, 0, stream>>>(a);
     else
         k_fill<MODE, CUT, HASHED, kFillChunks><<<nblocks, kBlock, 0, stream>>>(a);

@@ -278,12 +278,13 @@ def test_layout_follows_atom_order(gpu, order):
         _check(pos, cell, True, 4.2, msg=f"{order} build {rep}")
 
 
-@pytest.mark.parametrize("cut", ["global", "per_atom", "per_type", "f32"])
+@pytest.mark.parametrize("cut", ["global", "per_atom", "per_type", "f32", "matrix"])
 def test_five_chunk_fill_window(gpu, cut):
     """A density at which a home cell has 257-320 candidates (fcc 14^3,
     a = 3.615, rc 5: 10 cells a side, ~296 candidates), so the fill takes
     its 5-chunk window: uniform, per-atom and per-type cutoffs (dense and
-    hashed grids) bit for bit, float32 output to its tolerance."""
+    hashed grids) bit for bit, float32 output to its tolerance, and the dense
+    neighbour matrix (rows short of and above the largest count)."""
     a, reps = 3.615, 14
     basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
     g = np.stack(np.meshgrid(*[np.arange(reps)] * 3, indexing="ij"), -1).reshape(-1, 1, 3)
@@ -301,6 +302,19 @@ def test_five_chunk_fill_window(gpu, cut):
         types = rng.integers(0, 2, nat).astype(np.int32)
         _check(pos, cell, True, np.array([[4.0, 4.6], [4.6, 5.0]]), numbers=types,
                backends=(0, 1))
+    elif cut == "matrix":
+        from conftest import bits
+        from test_gpu_matrix import _engine_matrix
+        inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+        pb = np.ones(3, bool)
+        kmax = int(oracle.neighbour_matrix(np.zeros(3), cell, inv, pb, pos, 5.0, 1)[2].max())
+        for K in (kmax - 2, kmax + 3):
+            want = oracle.neighbour_matrix(np.zeros(3), cell, inv, pb, pos, 5.0, K)
+            for backend in (0, 1):
+                got = _engine_matrix(pos, cell, inv, pb, 5.0, None, K, backend)
+                assert np.array_equal(got[0], want[0]) and np.array_equal(got[2], want[2])
+                assert np.array_equal(bits(got[1]), bits(want[1]))
+                assert got[3] == want[3]
     else:
         ref = _oracle("ijdDS", pos, cell, True, 5.0)
         got = _engine("ijdDS", pos, cell, True, 5.0, f32=True)
