@@ -78,3 +78,14 @@ def test_argument_errors_return_status_without_device():
     assert lib.mnl_plan_pair_backward(None, None, 0, 0, None, None, None, None, None, None,
                                       None, 0, None, None) == -1
     assert b"plan is NULL" in lib.mnl_last_error()
+
+
+@pytest.mark.skipif(HAS_GPU, reason="checks the cache query on a host without a device")
+def test_result_cache_queries_without_device():
+    """The kept-buffer query and release touch no device when nothing is
+    kept: zero bytes for every device and for all, and a release is a no-op."""
+    from matscipy_neighbours_amd import _lib
+    lib = _lib.load()
+    assert lib.mnl_cached_bytes(-1) == 0 and lib.mnl_cached_bytes(0) == 0
+    lib.mnl_release_cached_memory()
+    assert lib.mnl_cached_bytes(-1) == 0
