@@ -393,7 +393,8 @@ def leg_autograd(lib, _lib, torch, dev, steps, warmup, with_cpu):
         _lib.check(lib.mnl_plan_pair_backward(step.plan.handle, step.sp, P, N,
                                               _lib.ptr(o["i"]), _lib.ptr(o["j"]), _lib.ptr(S),
                                               _lib.ptr(o["D"]), _lib.ptr(o["d"]), _lib.ptr(gD),
-                                              _lib.ptr(gd), 0, _lib.ptr(gpos), _lib.ptr(gcell)))
+                                              _lib.ptr(gd), 0, None, _lib.ptr(gpos),
+                                              _lib.ptr(gcell)))
     ms_b = _time_steps(torch, stream, bwd, steps, warmup)
     return "bcc_131k_autograd", {
         "atoms": N, "pairs": P,

@@ -329,9 +329,13 @@ void mnl_graph_destroy(mnl_graph *graph);
  * given dL/dD (grad_distvec [npairs*3], may be NULL) and dL/dd
  * (grad_absdist [npairs], may be NULL; then distvec and absdist are
  * required), writes dL/dr into grad_positions [nat*3] float64 and dL/dcell
- * (cell rows = lattice vectors, dL/dcell[a][b] = sum_p S_pa g_pb) into
- * grad_cell [9] float64, where g_p = dL/dD_p + dL/dd_p D_p / d_p (the d
- * term is dropped for d_p = 0). Both outputs are overwritten. distvec,
+ * (cell rows = lattice vectors, dL/dcell[a][b] = sum_p S_pa g_pb over the
+ * periodic directions a) into grad_cell [9] float64, where
+ * g_p = dL/dD_p + dL/dd_p D_p / d_p (the d term is dropped for d_p = 0).
+ * pbc [3] int32 [host] (NULL = all periodic) is the list's periodicity:
+ * along a non-periodic direction D = r_j - r_i does not involve the cell,
+ * even where the reference's S is non-zero there (an atom a box length or
+ * more outside). Both outputs are overwritten. distvec,
  * absdist and the incoming gradients are float32 when float32 != 0 (the FP32
  * output option), float64 otherwise. All pointers [device]; asynchronous.
  * The cell gradient is deterministic; grad_positions sums the j-side
@@ -342,7 +346,8 @@ int mnl_plan_pair_backward(mnl_plan *plan, void *stream, int64_t npairs, int32_t
                            const int32_t *shift, const void *distvec,
                            const void *absdist, const void *grad_distvec,
                            const void *grad_absdist, int float32,
-                           double *grad_positions, double *grad_cell);
+                           const int32_t pbc[3], double *grad_positions,
+                           double *grad_cell);
 
 /* ------------------------------------------------------------- autotuner
  *

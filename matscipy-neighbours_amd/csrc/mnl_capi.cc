@@ -116,6 +116,17 @@ int check_device_available() {
 
 bool valid_quantities(int q) { return (q & ~31) == 0; }
 
+/* The search range of a system: refused when negative (a negative cutoff)
+   or when the stencil has more than kMaxStencil neighbour-cell offsets. */
+const char *stencil_error(const SysGeom &g) {
+    const int64_t nseg = stencil_size(g.nxyz);
+    if (nseg < 0) return "negative cutoff";
+    if (nseg > kMaxStencil)
+        return "cutoff spans more than 2^30 neighbour cells of the grid "
+               "((2nx+1)(2ny+1)(2nz+1) with nx = ceil(cutoff n / width))";
+    return nullptr;
+}
+
 }  // namespace
 
 /* ======================================================================== */
@@ -369,11 +380,7 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
     SysGeom g{};
     if (!make_sys_geom(origin, cell, inv_cell, pbc, cutoff, 0, nat, 0, g))
         return fail("Zero cell volume.");
-    for (int k = 0; k < 3; k++) {
-        if (g.nxyz[k] > 255)
-            return fail("cutoff spans more than 255 cells of the grid in one direction");
-        if (g.nxyz[k] < 0) return fail("negative cutoff");
-    }
+    if (const char *why = stencil_error(g)) return fail(why);
     const int64_t ncells = static_cast<int64_t>(g.n[0]) * g.n[1] * g.n[2];
     const int64_t budget = std::max<int64_t>(int64_t(1) << 20, int64_t(8) * nat);
     bool hashed;
@@ -1122,12 +1129,8 @@ int mnl_plan_set_batch(mnl_plan *p, void *stream, int32_t nsys, const int32_t *a
                            inv_cells + 9 * int64_t(s), pbcs + 3 * int64_t(s), rc, a0, a1 - a0,
                            static_cast<int32_t>(ncells), g))
             return fail("Zero cell volume (system " + std::to_string(s) + ").");
-        for (int k = 0; k < 3; k++) {
-            if (g.nxyz[k] > 255)
-                return fail("cutoff spans more than 255 cells of the grid in one direction "
-                            "(system " + std::to_string(s) + ")");
-            if (g.nxyz[k] < 0) return fail("negative cutoff");
-        }
+        if (const char *why = stencil_error(g))
+            return fail(std::string(why) + " (system " + std::to_string(s) + ")");
         abase[static_cast<size_t>(s)] = a0;
         cbase[static_cast<size_t>(s)] = static_cast<int32_t>(ncells);
         ncells += static_cast<int64_t>(g.n[0]) * g.n[1] * g.n[2];
@@ -1277,7 +1280,8 @@ int mnl_plan_pair_backward(mnl_plan *p, void *stream, int64_t npairs, int32_t na
                            const int32_t *first, const int32_t *secnd, const int32_t *shift,
                            const void *distvec, const void *absdist,
                            const void *grad_distvec, const void *grad_absdist,
-                           int float32, double *grad_positions, double *grad_cell) {
+                           int float32, const int32_t pbc[3], double *grad_positions,
+                           double *grad_cell) {
     if (!p) return fail("plan is NULL");
     if (npairs < 0 || nat < 0) return fail("negative size");
     if (!grad_positions && nat > 0) return fail("grad_positions is NULL");
@@ -1301,8 +1305,9 @@ int mnl_plan_pair_backward(mnl_plan *p, void *stream, int64_t npairs, int32_t na
         MNL_HIP(p->cell_partial.ensure(need));
     }
     MNL_HIP(launch_pair_backward(npairs, nat, first, secnd, shift, distvec, absdist,
-                                 grad_distvec, grad_absdist, float32 != 0, grad_positions,
-                                 grad_cell, p->cell_partial.as<double>(), st));
+                                 grad_distvec, grad_absdist, float32 != 0,
+                                 pbc ? (pbc[0] ? 1 : 0) | (pbc[1] ? 2 : 0) | (pbc[2] ? 4 : 0) : 7,
+                                 grad_positions, grad_cell, p->cell_partial.as<double>(), st));
     return MNL_SUCCESS;
 }
 

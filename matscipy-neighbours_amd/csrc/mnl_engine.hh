@@ -63,6 +63,14 @@ void make_div_magic(int32_t d, uint32_t &mul, int32_t &shift);
 bool grid_geometry(const double cell[9], double cutoff, int32_t n[3],
                    int32_t nxyz[3], double bin[9], double len[3]);
 
+/* Neighbour-cell offsets one home cell visits, (2nx+1)(2ny+1)(2nz+1), or -1
+   for a negative reach; values above kMaxStencil come back as
+   kMaxStencil + 1. The pair kernels index the stencil with int32 and need
+   it at most kMaxStencil (2^30, far beyond any system that finishes:
+   each home atom then tests over 10^9 cells). */
+constexpr int64_t kMaxStencil = int64_t(1) << 30;
+int64_t stencil_size(const int32_t nxyz[3]);
+
 /* Fill a SysGeom for one system. Returns false for a zero-volume cell. */
 bool make_sys_geom(const double origin[3], const double cell[9],
                    const double inv_cell[9], const int32_t pbc[3], double cutoff,
@@ -202,7 +210,9 @@ hipError_t launch_sqrt_probe(const double *x, double *y, int64_t n,
                              hipStream_t stream);
 /* Backward of D = r[j] - r[i] + S.cell, d = |D| (see mnl_kernels.hip):
    overwrites grad_pos [nat*3] and grad_cell [9]; cell_partial is scratch of
-   backward_scratch_doubles(nat) doubles. D, d, gD, gd are float when f32. */
+   backward_scratch_doubles(nat) doubles. D, d, gD, gd are float when f32.
+   pbc_mask: bit k set if direction k is periodic (S enters the cell
+   gradient only there). */
 constexpr int kBackwardMaxBlocks = 2048;
 int backward_blocks(int64_t npairs);
 inline int64_t backward_scratch_doubles(int32_t nat) {
@@ -211,8 +221,8 @@ inline int64_t backward_scratch_doubles(int32_t nat) {
 hipError_t launch_pair_backward(int64_t npairs, int32_t nat, const int32_t *first,
                                 const int32_t *secnd, const int32_t *shift, const void *D,
                                 const void *d, const void *gD, const void *gd, bool f32,
-                                double *grad_pos, double *grad_cell, double *cell_partial,
-                                hipStream_t stream);
+                                int pbc_mask, double *grad_pos, double *grad_cell,
+                                double *cell_partial, hipStream_t stream);
 /* out[s] = offsets[atom_base[s]] for s = 0..nsys: each system's first pair. */
 hipError_t launch_system_offsets(const int64_t *offsets, const int32_t *atom_base,
                                  int32_t nsys, int64_t *out, hipStream_t stream);

@@ -24,6 +24,15 @@ void cross3(const double *a, const double *b, double *c) {
 /* |a| (src/libneighbours/tools.cc:23-25). */
 double norm3(const double *a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
 
+/* The reference converts floor(width / cutoff) and ceil(cutoff n / width)
+   with static_cast<index_t>; on x86-64 an out-of-range or NaN value gives
+   INT_MIN (cvttsd2si), which its max(., 1) turns into one cell (cutoff 0, a
+   width far above the cutoff). The same rule is spelled out here so that no
+   conversion is undefined. */
+int32_t to_index(double v) {
+    return (v >= -2147483648.0 && v < 2147483648.0) ? static_cast<int32_t>(v) : INT32_MIN;
+}
+
 }  // namespace
 
 /* src/libneighbours/cell_list.cc:153-182: volume from the triple product,
@@ -42,7 +51,7 @@ bool grid_geometry(const double cell[9], double cutoff, int32_t n[3],
     if (!(volume >= 1e-12)) return false;
     for (int k = 0; k < 3; k++) {
         len[k] = volume / norm3(nrm[k]);
-        const int32_t nk = static_cast<int32_t>(std::floor(len[k] / cutoff));
+        const int32_t nk = to_index(std::floor(len[k] / cutoff));
         n[k] = nk > 1 ? nk : 1;
     }
     for (int m = 0; m < 3; m++) {
@@ -50,8 +59,11 @@ bool grid_geometry(const double cell[9], double cutoff, int32_t n[3],
         bin[3 + m] = c2[m] / n[1];
         bin[6 + m] = c3[m] / n[2];
     }
-    for (int k = 0; k < 3; k++)
-        nxyz[k] = static_cast<int32_t>(std::ceil(cutoff * n[k] / len[k]));
+    for (int k = 0; k < 3; k++) {
+        /* a reach beyond int32 is refused by the caller (stencil_size) */
+        const double reach = std::ceil(cutoff * n[k] / len[k]);
+        nxyz[k] = reach >= 2147483647.0 ? INT32_MAX : to_index(reach);
+    }
     return true;
 }
 
@@ -73,12 +85,26 @@ bool make_sys_geom(const double origin[3], const double cell[9],
     g.cell_base = cell_base;
     const int64_t nc = static_cast<int64_t>(g.n[0]) * g.n[1] * g.n[2];
     g.ncells = nc > INT32_MAX ? INT32_MAX : static_cast<int32_t>(nc);
-    g.nseg = (2 * g.nxyz[0] + 1) * (2 * g.nxyz[1] + 1) * (2 * g.nxyz[2] + 1);
+    const int64_t nseg = stencil_size(g.nxyz);
+    g.nseg = nseg > INT32_MAX ? INT32_MAX : static_cast<int32_t>(nseg);
     for (int k = 0; k < 3; k++) g.lay[k] = k;
-    const int32_t divisors[5] = {g.n[0], g.n[1], g.n[2], 2 * g.nxyz[0] + 1,
-                                 2 * g.nxyz[1] + 1};
+    int32_t divisors[5] = {g.n[0], g.n[1], g.n[2], 1, 1};
+    if (nseg <= kMaxStencil) {
+        divisors[3] = 2 * g.nxyz[0] + 1;
+        divisors[4] = 2 * g.nxyz[1] + 1;
+    }
     for (int k = 0; k < 5; k++) make_div_magic(divisors[k], g.dmul[k], g.dshift[k]);
     return true;
+}
+
+int64_t stencil_size(const int32_t nxyz[3]) {
+    int64_t s = 1;
+    for (int k = 0; k < 3; k++) {
+        if (nxyz[k] < 0) return -1;
+        s *= 2 * static_cast<int64_t>(nxyz[k]) + 1;
+        if (s > kMaxStencil) return kMaxStencil + 1;
+    }
+    return s;
 }
 
 /* s = ceil(log2 d), mul = floor(2^(31+s) / d) + 1 < 2^32. For x < 2^31,

@@ -546,16 +546,6 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const TI *__restrict__ in
 
 enum { MODE_COUNT = 0, MODE_FILL64 = 1, MODE_FILL32 = 2, MODE_MATRIX = 3 };
 
-/* Pack three small signed integers (|v| < 512) into 30 bits and back. */
-__device__ __forceinline__ int pack3(int a, int b, int c) {
-    return ((a + 512) & 1023) | (((b + 512) & 1023) << 10) | (((c + 512) & 1023) << 20);
-}
-__device__ __forceinline__ void unpack3(int p, int &a, int &b, int &c) {
-    a = (p & 1023) - 512;
-    b = ((p >> 10) & 1023) - 512;
-    c = ((p >> 20) & 1023) - 512;
-}
-
 /* Dense slice of cell (c0,c1,c2) of a system (cell_list.hh:94-104). */
 __device__ __forceinline__ void dense_slice(const PairArgs &A, int cell_base, int lf, int lm,
                                             int ls, int nf, int nm, int c0, int c1, int c2,
@@ -742,8 +732,17 @@ struct SegLds {
     double so[3][kWave]; /* offset vector off = (z b3 + y b2) + x b1 */
     int st[kWave];       /* first candidate index of each segment */
     int sb[kWave];       /* first cell-sorted atom of each segment */
-    int sxyz[kWave];     /* packed cell offset (x, y, z) */
 };
+
+/* Cell offset (x, y, z) of segment `seg` (x fastest, each -n..n). */
+__device__ __forceinline__ void segment_offset(const HomeCell &H, int seg, int &x, int &y,
+                                               int &z) {
+    const int rest = div_magic(seg, H.msx, H.hsx);
+    x = seg - rest * H.sx - H.nx;
+    const int zq = div_magic(rest, H.msy, H.hsy);
+    y = rest - zq * H.sy - H.ny;
+    z = zq - H.nz;
+}
 
 /* Builds the table of segments sg .. sg+nsg-1; returns the group's candidate
    count, and in *self_start the candidate index of the home cell's first atom
@@ -751,14 +750,10 @@ struct SegLds {
 template <bool HASHED>
 __device__ __forceinline__ int build_segments(const PairArgs &A, const HomeCell &H, int sg,
                                               int nsg, int lane, SegLds &S, int *self_start) {
-    int len = 0, begin = 0, pxyz = 0;
+    int len = 0, begin = 0;
     if (lane < nsg) {
-        const int seg = sg + lane;
-        const int rest = div_magic(seg, H.msx, H.hsx);
-        const int x = seg - rest * H.sx - H.nx;
-        const int zq = div_magic(rest, H.msy, H.hsy);
-        const int y = rest - zq * H.sy - H.ny;
-        const int z = zq - H.nz;
+        int x, y, z;
+        segment_offset(H, sg + lane, x, y, z);
         int cj0 = H.c0 + x, cj1 = H.c1 + y, cj2 = H.c2 + z;
         bool ok = true;
         if (H.p0)
@@ -780,7 +775,6 @@ __device__ __forceinline__ int build_segments(const PairArgs &A, const HomeCell 
                 dense_slice(A, H.cbase, H.lf, H.lm, H.ls, H.nf, H.nm, cj0, cj1, cj2, begin,
                             len);
         }
-        pxyz = pack3(x, y, z);
         /* the offset vector, neighbour_visit.hh:67,73-74,80-81 */
         S.so[0][lane] = (z * H.b20 + y * H.b10) + x * H.b00;
         S.so[1][lane] = (z * H.b21 + y * H.b11) + x * H.b01;
@@ -792,7 +786,6 @@ __device__ __forceinline__ int build_segments(const PairArgs &A, const HomeCell 
     *self_start = (ci >= 0 && ci < nsg) ? uniform(__shfl(incl - len, ci, kWave)) : -1;
     S.st[lane] = incl - len;
     S.sb[lane] = begin;
-    S.sxyz[lane] = pxyz;
     wave_sync_lds();
     return total;
 }
@@ -820,13 +813,13 @@ struct Cand {
     int c0, c1, c2;        /* shift numerator part x - raw_j (fill only) */
 };
 
-/* Candidate c of the group's candidate space: its segment (the last one
-   starting at or before c; empty segments never win), cell-sorted index,
-   record and the segment's offset vector. */
+/* Candidate c of the group of segments from sg on: its segment (the last
+   one starting at or before c; empty segments never win), cell-sorted
+   index, record and the segment's offset vector. */
 template <int CUT, bool FILL>
 __device__ __forceinline__ void load_candidate(const PairArgs &A, const HomeCell &H,
-                                               const SegLds &S, int nsg, int c, int total,
-                                               Cand &C) {
+                                               const SegLds &S, int sg, int nsg, int c,
+                                               int total, Cand &C) {
     const bool cv = c < total;
     const int p = last_at_or_below(S.st, nsg, c);
     const int sj = cv ? S.sb[p] + (c - S.st[p]) : H.hb;
@@ -841,7 +834,7 @@ __device__ __forceinline__ void load_candidate(const PairArgs &A, const HomeCell
     C.tj = (CUT == CUT_PER_TYPE || CUT == CUT_PER_TYPE_LDS) ? A.stype[sj] : 0;
     if (FILL) {
         int x, y, z;
-        unpack3(S.sxyz[p], x, y, z);
+        segment_offset(H, sg + p, x, y, z);
         const int4 bj = A.sinfo[sj];
         C.j = bj.w;
         C.c0 = x - bj.x;
@@ -909,7 +902,7 @@ __device__ __forceinline__ void count_cell(const PairArgs &A, int h, int part, S
             const int self_base = self_start >= 0 ? self_start + grp0 : -(1 << 30);
             for (int cb = 0; cb < total; cb += kWave) {
                 Cand C;
-                load_candidate<CUT, false>(A, H, S, nsg, cb + lane, total, C);
+                load_candidate<CUT, false>(A, H, S, sg, nsg, cb + lane, total, C);
                 const int c = cb + lane;
                 uint32_t mlo = 0, mhi = 0;
                 for (int t = 0; t < mg; t++) {
@@ -996,7 +989,7 @@ __device__ __forceinline__ void count_cell_half(const PairArgs &A, int h, int pa
             const int total = build_segments<HASHED>(A, H, sg, nsg, lane, S, &self_start);
             for (int cb = 0; cb < total; cb += kWave) {
                 Cand C;
-                load_candidate<CUT, false>(A, H, S, nsg, cb + lane, total, C);
+                load_candidate<CUT, false>(A, H, S, sg, nsg, cb + lane, total, C);
                 uint32_t icnt = 0;
                 int cj = 0;
                 /* the centre segment leads the first group: its candidates
@@ -1014,10 +1007,11 @@ __device__ __forceinline__ void count_cell_half(const PairArgs &A, int h, int pa
     }
 }
 
-/* Per-type cutoff tables up to this many entries are staged in LDS once per
-   block (the test reads one entry per candidate and home atom); larger ones
-   are read from global memory. */
-constexpr int kLdsTypes = 256;
+/* Per-type cutoff tables up to this many entries (11 types) are staged in
+   LDS once per block (the test reads one entry per candidate and home atom);
+   larger ones are read from global memory. At 128 entries four fill blocks
+   still fit in a CU's 160 KiB of LDS. */
+constexpr int kLdsTypes = 128;
 
 template <int CUT>
 __device__ __forceinline__ lds_dptr stage_types(const PairArgs &A, double *s_pt) {
@@ -1085,7 +1079,7 @@ __device__ __forceinline__ void lj_cell(const PairArgs &A, int h, int part, SegL
             const int self_base = self_start >= 0 ? self_start + grp0 : -(1 << 30);
             for (int cb = 0; cb < total; cb += kWave) {
                 Cand C;
-                load_candidate<CUT_GLOBAL, false>(A, H, S, nsg, cb + lane, total, C);
+                load_candidate<CUT_GLOBAL, false>(A, H, S, sg, nsg, cb + lane, total, C);
                 const int c = cb + lane;
                 double f0 = 0.0, f1 = 0.0, f2 = 0.0;
                 for (int t = 0; t < mg; t++) {
@@ -1204,7 +1198,7 @@ template <int MODE>
 __device__ __forceinline__ void write_pair(const PairArgs &A, int64_t pos, int i, int j,
                                            double dx, double dy, double dz, int s0, int s1,
                                            int s2) {
-    if (pos >= A.capacity) return;
+    if (pos < 0 || pos >= A.capacity) return;
     const double r2 = dx * dx + dy * dy + dz * dz;
     if (A.out_i) A.out_i[pos] = i;
     if (A.out_j) A.out_j[pos] = j;
@@ -1235,13 +1229,15 @@ __device__ __forceinline__ void write_pair(const PairArgs &A, int64_t pos, int i
 
 /* Fill-pass LDS of one wave: segment table, the home atoms of the current
    group (output row, image/raw cell, index) and the ring of accepted pairs
-   (dr, j, shift numerator part, atom and rank in its row). */
+   (dr, j, shift numerator part, home atom slot, rank in the atom's row).
+   The rank has a word of its own: a row may hold up to 2^31 - 1 pairs, as
+   the reference's int32 offsets allow. */
 struct FillLds {
     SegLds seg;
     int64_t hpos[kWave];
     int hbx[kWave], hby[kWave], hbz[kWave], hi[kWave];
     double rdx[kRing], rdy[kRing], rdz[kRing];
-    int rj[kRing], rc0[kRing], rc1[kRing], rc2[kRing], rtk[kRing];
+    int rj[kRing], rc0[kRing], rc1[kRing], rc2[kRing], rt[kRing], rrank[kRing];
 };
 
 /* Write n (<= 64) ring entries from `tail` on, one per lane. */
@@ -1252,13 +1248,12 @@ __device__ __forceinline__ void flush_ring(const PairArgs &A, const HomeCell &H,
     wave_sync_lds();
     if (lane < n) {
         const int e = (tail + lane) & (kRing - 1);
-        const int tk = L.rtk[e];
-        const int t = tk & (kWave - 1);
-        const int rank = tk >> 6;
+        const int t = L.rt[e] & (kWave - 1);
+        const int rank = L.rrank[e];
         const int64_t pos = L.hpos[t] + rank;
         if (MODE == MODE_MATRIX) {
             /* row of the dense matrix: neighbour index and D, ranks < K */
-            if (rank < A.max_nb) {
+            if (rank >= 0 && rank < A.max_nb) {
                 A.out_j[pos] = L.rj[e];
                 double *D = static_cast<double *>(A.out_D) + 3 * pos;
                 D[0] = L.rdx[e];
@@ -1320,7 +1315,7 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, int part, Fi
 #pragma unroll
                 for (int k = 0; k < FC; k++)
                     if (w0 + k * kWave < total)
-                        load_candidate<CUT, true>(A, H, L.seg, nsg, w0 + k * kWave + lane,
+                        load_candidate<CUT, true>(A, H, L.seg, sg, nsg, w0 + k * kWave + lane,
                                                   total, C[k]);
                 for (int t = 0; t < mg; t++) {
                     const int si = H.hb + grp0 + t;
@@ -1348,7 +1343,8 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, int part, Fi
                                 L.rc0[e] = C[k].c0;
                                 L.rc1[e] = C[k].c1;
                                 L.rc2[e] = C[k].c2;
-                                L.rtk[e] = t | ((rank0 + r) << 6);
+                                L.rt[e] = t;
+                                L.rrank[e] = rank0 + r;
                             }
                             const int pc = static_cast<int>(__popcll(mask));
                             rank0 += pc;
@@ -1613,7 +1609,9 @@ constexpr int kHashProbe = 8;
  *     straight to global memory;
  *   dL/dr[i] -= g summed first over each i-segment inside the wave (one
  *     atomic per segment per 64 pairs);
- *   dL/dcell[a][b] = sum_p S_pa g_pb kept per thread, reduced per block into
+ *   dL/dcell[a][b] = sum_p S_pa g_pb over the periodic directions a (along
+ *     a non-periodic one D = r_j - r_i does not involve the cell, whatever
+ *     S holds there), kept per thread, reduced per block into
  *     cell_partial[block][9] (fixed tiles, fixed order: deterministic).
  * The position sums go to separate x, y, z arrays (gx, gy, gz).
  */
@@ -1622,7 +1620,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_backward(
     int64_t npairs, int64_t tile, const int32_t *__restrict__ first,
     const int32_t *__restrict__ secnd, const int32_t *__restrict__ shift,
     const T *__restrict__ D, const T *__restrict__ d, const T *__restrict__ gD,
-    const T *__restrict__ gd, double *__restrict__ gx, double *__restrict__ gy,
+    const T *__restrict__ gd, int pbc_mask, double *__restrict__ gx, double *__restrict__ gy,
     double *__restrict__ gz, double *__restrict__ cell_partial) {
     __shared__ double s_red[kWavesPerBlock][9];
     __shared__ int s_key[kHashSlots];
@@ -1664,9 +1662,10 @@ __global__ __launch_bounds__(kBlock) void k_pair_backward(
                     g2 += f * static_cast<double>(D[3 * p + 2]);
                 }
             }
-            s0 = static_cast<double>(shift[3 * p + 0]);
-            s1 = static_cast<double>(shift[3 * p + 1]);
-            s2 = static_cast<double>(shift[3 * p + 2]);
+            /* the cell enters D only along periodic directions */
+            s0 = (pbc_mask & 1) ? static_cast<double>(shift[3 * p + 0]) : 0.0;
+            s1 = (pbc_mask & 2) ? static_cast<double>(shift[3 * p + 1]) : 0.0;
+            s2 = (pbc_mask & 4) ? static_cast<double>(shift[3 * p + 2]) : 0.0;
             /* j-side: the block's table, or global memory when full */
             const unsigned hsh = static_cast<unsigned>(j) * 2654435761u;
             int slot = -1;
@@ -2022,8 +2021,8 @@ static int64_t backward_tile(int64_t npairs, int nb) {
 hipError_t launch_pair_backward(int64_t npairs, int32_t nat, const int32_t *first,
                                 const int32_t *secnd, const int32_t *shift, const void *D,
                                 const void *d, const void *gD, const void *gd, bool f32,
-                                double *grad_pos, double *grad_cell, double *cell_partial,
-                                hipStream_t stream) {
+                                int pbc_mask, double *grad_pos, double *grad_cell,
+                                double *cell_partial, hipStream_t stream) {
     const int nb = backward_blocks(npairs);
     /* per-component sums after the cell partials */
     double *soa = cell_partial + 9 * static_cast<int64_t>(kBackwardMaxBlocks);
@@ -2034,13 +2033,13 @@ hipError_t launch_pair_backward(int64_t npairs, int32_t nat, const int32_t *firs
             k_pair_backward<float><<<nb, kBlock, 0, stream>>>(
                 npairs, backward_tile(npairs, nb), first, secnd, shift, static_cast<const float *>(D),
                 static_cast<const float *>(d), static_cast<const float *>(gD),
-                static_cast<const float *>(gd), soa, soa + nat, soa + 2 * int64_t(nat),
+                static_cast<const float *>(gd), pbc_mask, soa, soa + nat, soa + 2 * int64_t(nat),
                 cell_partial);
         else
             k_pair_backward<double><<<nb, kBlock, 0, stream>>>(
                 npairs, backward_tile(npairs, nb), first, secnd, shift, static_cast<const double *>(D),
                 static_cast<const double *>(d), static_cast<const double *>(gD),
-                static_cast<const double *>(gd), soa, soa + nat, soa + 2 * int64_t(nat),
+                static_cast<const double *>(gd), pbc_mask, soa, soa + nat, soa + 2 * int64_t(nat),
                 cell_partial);
         k_cell_reduce<<<1, kBlock, 0, stream>>>(cell_partial, nb, grad_cell);
     } else {

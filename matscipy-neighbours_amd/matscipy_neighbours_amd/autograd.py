@@ -7,7 +7,8 @@ D_p = r[j] - r[i] + S_p @ cell and d_p = |D_p|:
 
     g_p           = dL/dD_p + dL/dd_p * D_p / d_p
     dL/dr[j]     += g_p,   dL/dr[i] -= g_p
-    dL/dcell[a,b] = sum_p S_pa g_pb
+    dL/dcell[a,b] = sum_p S_pa g_pb   (periodic directions a only: along a
+                                       non-periodic one D = r_j - r_i)
 
 The pair set (i, j, S) is piecewise constant in the positions and the cell,
 so it carries no gradient, exactly as for a list built on the CPU.
@@ -40,6 +41,7 @@ def _function():
             out = b.run(31, f32=f32)
             i, j, D, d, S = out["i"], out["j"], out["D"], out["d"], out["S"]
             ctx.save_for_backward(i, j, S, D, d)
+            ctx.pbc = np.ascontiguousarray(pbc, dtype=np.int32)
             ctx.nat = int(pos.shape[0])
             ctx.f32 = f32
             ctx.pos_dtype = positions.dtype
@@ -68,7 +70,8 @@ def _function():
                 _lib.check(lib.mnl_plan_pair_backward(
                     plan.handle, sp, int(i.shape[0]), ctx.nat, _lib.ptr(i), _lib.ptr(j),
                     _lib.ptr(S), _lib.ptr(D), _lib.ptr(d), _lib.ptr(gD), _lib.ptr(gd),
-                    1 if ctx.f32 else 0, _lib.ptr(grad_pos), _lib.ptr(grad_cell)))
+                    1 if ctx.f32 else 0, _lib.ptr(ctx.pbc), _lib.ptr(grad_pos),
+                    _lib.ptr(grad_cell)))
             gp = grad_pos.to(ctx.pos_dtype) if need_pos else None
             gc = grad_cell.to(ctx.cell_dtype) if need_cell else None
             return gp, gc, None, None, None, None

@@ -185,3 +185,37 @@ def test_gradcheck_small_triclinic(gpu):
         return D, d
 
     assert torch.autograd.gradcheck(f, (pos, cell), eps=1e-6, atol=1e-7, rtol=1e-6)
+
+
+def test_cell_gradient_ignores_nonperiodic_shift(gpu):
+    """pbc (T, T, F) with atoms more than a box length outside along z: the
+    reference's S then holds the home atom's image count there (its
+    truncating (raw_i - rel_j + x)/n, neighbour_visit.hh:111-114), while D
+    is r_j - r_i along z. The cell gradient follows D, i.e. torch autograd
+    of r[j] - r[i] + (S * pbc) @ cell."""
+    import torch
+
+    from matscipy_neighbours_amd.autograd import neighbour_list_autograd
+    rng = np.random.default_rng(21)
+    cellnp = np.array([[6.0, 0.0, 0.0], [0.8, 5.7, 0.0], [0.3, -0.4, 6.2]])
+    frac = rng.uniform(0, 1, (60, 3))
+    frac[:20, 2] = rng.uniform(-2.6, -1.2, 20)   # one to two and a half boxes below
+    frac[20:40, 2] = rng.uniform(2.2, 3.5, 20)   # above
+    posnp = frac @ cellnp
+    pbc = (True, True, False)
+    pos = torch.tensor(posnp, dtype=torch.float64, device="cuda:0", requires_grad=True)
+    cell = torch.tensor(cellnp, dtype=torch.float64, device="cuda:0", requires_grad=True)
+    i, j, D, d, S = neighbour_list_autograd("ijDdS", pos, cell, pbc, 3.0)
+    assert int((S[:, 2] != 0).sum()) > 0, "no pair carries a non-periodic image count"
+    w = torch.randn(D.shape, dtype=torch.float64, device="cuda:0",
+                    generator=torch.Generator(device="cuda:0").manual_seed(5))
+    L = _loss(d) + (w * D).sum()
+    L.backward()
+    mask = torch.tensor([1.0, 1.0, 0.0], dtype=torch.float64, device="cuda:0")
+    p = pos.detach().clone().requires_grad_(True)
+    c = cell.detach().clone().requires_grad_(True)
+    Dr = p[j.long()] - p[i.long()] + (S.to(torch.float64) * mask) @ c
+    assert _rel_err(D.detach(), Dr.detach()) < 1e-12
+    (_loss(torch.linalg.norm(Dr, dim=1)) + (w * Dr).sum()).backward()
+    assert _rel_err(pos.grad, p.grad) < 1e-10
+    assert _rel_err(cell.grad, c.grad) < 1e-10
