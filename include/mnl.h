@@ -261,6 +261,32 @@ int mnl_plan_matrix_async(mnl_plan *plan, void *stream, const double origin[3],
                           int32_t *idx, double *dist, int32_t *count,
                           int32_t *overflow_device);
 
+/*
+ * Allocation-free build: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). One call does what
+ * mnl_plan_count_async followed by mnl_plan_fill_async does, on the current
+ * device and on `stream`, but every workspace lives in a fixed 24 MiB device
+ * array inside the library (each GPU the library runs on holds one until the
+ * process ends), so it calls no hipMalloc or hipFree and never waits for the
+ * device. Limits: nat <= 131072 and at most 524288 dense cells (or hash
+ * slots); larger builds are refused with an error. Cutoffs: the scalar
+ * cutoff, or per-atom radii per_atom_cutoff [device] (pair cutoff = sum of
+ * radii, binning cutoff = `cutoff`); no per-type tables. Pairs at positions
+ * >= capacity are not written; the pair total goes to *npairs_device
+ * ([device] int64, may be NULL), as in mnl_plan_fill_async. D and d are
+ * float32 when float32_output != 0. Output order and bits are those of the
+ * plan API. All calls on one GPU share its array, which keeps the last
+ * call's contents: calls that overlap on one GPU (other streams, other
+ * threads) race; order them on one stream.
+ */
+int mnl_neighbour_list_noalloc(void *stream, const double origin[3], const double cell[9],
+                               const double inv_cell[9], const int32_t pbc[3], int32_t nat,
+                               const double *positions, double cutoff,
+                               const double *per_atom_cutoff, int backend, int64_t capacity,
+                               int32_t *first, int32_t *secnd, void *distvec, void *absdist,
+                               int32_t *shift, int float32_output, int64_t *npairs_device);
+
 /* ---------------------------------------------------------------- batches
  *
  * Many independent systems in one build: atoms concatenated in system

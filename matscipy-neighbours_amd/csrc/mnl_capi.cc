@@ -165,6 +165,9 @@ struct mnl_plan {
     bool tri_all = false; /* no distance cutoff: every pair kept, no index table */
     /* batched geometry uploaded by mnl_plan_set_batch and still in place */
     bool batch_ready = false;
+    /* workspaces carved from a fixed device array (the allocation-free
+       build): they never grow, a build that needs more is refused */
+    bool fixed = false;
 };
 
 namespace {
@@ -280,6 +283,10 @@ int ensure_workspaces(mnl_plan *p, hipStream_t stream, int32_t nat, int64_t cell
     bool grow = false;
     for (const Need &nd : needs) grow = grow || nd.bytes > nd.b->cap;
     if (!grow) return MNL_SUCCESS;
+    if (p->fixed)
+        return fail("the build exceeds the allocation-free workspace (at most " +
+                    std::to_string(kNoallocMaxAtoms) + " atoms and " +
+                    std::to_string(kNoallocMaxCells) + " cells or hash slots)");
     hipStreamCaptureStatus cap_status = hipStreamCaptureStatusNone;
     MNL_HIP(hipStreamIsCapturing(stream, &cap_status));
     if (cap_status != hipStreamCaptureStatusNone)
@@ -513,6 +520,101 @@ mnl_plan *shared_plan(int device) {
 }
 
 }  // namespace
+
+/* ------------------------------------------------ allocation-free build */
+
+namespace {
+
+constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+/* Bytes of every workspace at the allocation-free limits, in carve order
+   (the order of ensure_workspaces' list). */
+constexpr size_t kNoallocPartials = 1024; /* >= scan_partial_size(kNoallocMaxCells) */
+constexpr size_t noalloc_sizes[] = {
+    size_t(kNoallocMaxAtoms) * sizeof(int4),          /* raw4 */
+    size_t(kNoallocMaxAtoms) * sizeof(int32_t),       /* ticket */
+    size_t(kNoallocMaxAtoms) * sizeof(int32_t),       /* tmp */
+    size_t(kNoallocMaxAtoms) * sizeof(double4),       /* spos */
+    size_t(kNoallocMaxAtoms) * sizeof(int4),          /* sinfo */
+    size_t(kNoallocMaxAtoms) * sizeof(int32_t),       /* scount */
+    size_t(kNoallocMaxAtoms) * sizeof(int32_t),       /* slot */
+    size_t(kNoallocMaxAtoms) * sizeof(int32_t),       /* counts */
+    (size_t(kNoallocMaxAtoms) + 1) * sizeof(int64_t), /* offsets */
+    size_t(kNoallocMaxCells) * sizeof(int32_t),       /* cell_count */
+    size_t(kNoallocMaxCells) * sizeof(int32_t),       /* cell_first */
+    kNoallocPartials * sizeof(int64_t),               /* partial */
+    sizeof(SysGeom),                                  /* geo */
+    2 * sizeof(int32_t),                              /* atom_base */
+    2 * sizeof(int32_t),                              /* cell_base */
+    sizeof(int64_t),                                  /* total */
+    size_t(kNoallocMaxCells) * sizeof(int64_t),       /* hkey */
+    3 * sizeof(int32_t),                              /* order_stats */
+};
+
+constexpr size_t noalloc_total() {
+    size_t off = 0;
+    for (size_t b : noalloc_sizes) off = align256(off) + b;
+    return off;
+}
+static_assert(noalloc_total() <= kNoallocScratchBytes,
+              "the allocation-free workspaces exceed the device array");
+
+std::mutex g_fixed_mutex;
+mnl_plan g_fixed_plans[kNoallocMaxDevices];
+
+/* Point the plan's workspaces into scratch; they never grow. */
+void carve_fixed_plan(mnl_plan *p, int device, void *scratch) {
+    DevBuf *bufs[] = {&p->raw4,   &p->ticket,     &p->tmp,        &p->spos,    &p->sinfo,
+                      &p->scount, &p->slot,       &p->counts,     &p->offsets, &p->cell_count,
+                      &p->cell_first, &p->partial, &p->geo,       &p->atom_base,
+                      &p->cell_base,  &p->total,   &p->hkey,      &p->order_stats};
+    static_assert(sizeof(bufs) / sizeof(bufs[0]) ==
+                      sizeof(noalloc_sizes) / sizeof(noalloc_sizes[0]),
+                  "one size per carved workspace");
+    char *base = static_cast<char *>(scratch);
+    size_t off = 0;
+    for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); k++) {
+        off = align256(off);
+        bufs[k]->ptr = base + off;
+        bufs[k]->cap = noalloc_sizes[k];
+        off += noalloc_sizes[k];
+    }
+    p->device = device;
+    p->fixed = true;
+}
+
+}  // namespace
+
+int mnl::set_error(const char *msg) { return fail(msg); }
+
+int mnl::noalloc_build(int device, void *scratch, hipStream_t stream, const double origin[3],
+                       const double cell[9], const double inv_cell[9], const int32_t pbc[3],
+                       int32_t nat, const double *positions, double cutoff,
+                       const double *per_atom_cutoff, int backend, int64_t capacity,
+                       int32_t *first, int32_t *secnd, void *distvec, void *absdist,
+                       int32_t *shift, int float32_output, int64_t *npairs_device) {
+    if (device < 0 || device >= kNoallocMaxDevices)
+        return fail("device index beyond the allocation-free build's table");
+    if (!origin || !cell || !inv_cell || !pbc) return fail("geometry pointer is NULL");
+    if (nat > 0 && !positions) return fail("positions is NULL");
+    if (capacity < 0) return fail("negative capacity");
+    if (nat > kNoallocMaxAtoms)
+        return fail("the allocation-free build takes at most " +
+                    std::to_string(kNoallocMaxAtoms) + " atoms");
+    std::lock_guard<std::mutex> lock(g_fixed_mutex);
+    mnl_plan *p = &g_fixed_plans[device];
+    if (!p->fixed) carve_fixed_plan(p, device, scratch);
+    if (enqueue_count(p, stream, origin, cell, inv_cell, pbc, nat, positions, cutoff,
+                      per_atom_cutoff, nullptr, 0, nullptr, backend, true,
+                      npairs_device) != MNL_SUCCESS)
+        return MNL_ERROR;
+    p->counted = true;
+    p->npairs = capacity;
+    const int st = enqueue_fill(p, stream, float32_output ? 2 : 1, first, secnd, distvec,
+                                absdist, shift);
+    p->npairs = -1;
+    return st;
+}
 
 /* ======================================================================== */
 

@@ -223,6 +223,28 @@ hipError_t launch_pair_backward(int64_t npairs, int32_t nat, const int32_t *firs
                                 const void *d, const void *gD, const void *gd, bool f32,
                                 int pbc_mask, double *grad_pos, double *grad_cell,
                                 double *cell_partial, hipStream_t stream);
+/* The allocation-free build (mnl_neighbour_list_noalloc): its limits and
+   the size of the per-device array (mnl_noalloc.hip) that its workspaces
+   are carved from. */
+constexpr int32_t kNoallocMaxAtoms = 1 << 17;
+constexpr int64_t kNoallocMaxCells = int64_t(1) << 19; /* dense cells or hash slots */
+constexpr size_t kNoallocScratchBytes = size_t(24) << 20;
+constexpr int kNoallocMaxDevices = 64;
+
+/* Records msg as the calling thread's last error; returns -1 (MNL_ERROR). */
+int set_error(const char *msg);
+
+/* Count and fill of one system (mnl_plan_count_async followed by
+   mnl_plan_fill_async, uniform or per-atom cutoffs) with every workspace
+   inside `scratch` (device memory of kNoallocScratchBytes on `device`, the
+   current device): no allocation, no host synchronisation. */
+int noalloc_build(int device, void *scratch, hipStream_t stream, const double origin[3],
+                  const double cell[9], const double inv_cell[9], const int32_t pbc[3],
+                  int32_t nat, const double *positions, double cutoff,
+                  const double *per_atom_cutoff, int backend, int64_t capacity,
+                  int32_t *first, int32_t *secnd, void *distvec, void *absdist,
+                  int32_t *shift, int float32_output, int64_t *npairs_device);
+
 /* out[s] = offsets[atom_base[s]] for s = 0..nsys: each system's first pair. */
 hipError_t launch_system_offsets(const int64_t *offsets, const int32_t *atom_base,
                                  int32_t nsys, int64_t *out, hipStream_t stream);

@@ -54,6 +54,9 @@ SIGNATURES = [
     ("mnl_plan_fill_async", ctypes.c_int,
      [_vp, _vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]),
     ("mnl_plan_fill", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("mnl_neighbour_list_noalloc", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp, ctypes.c_double, _vp, ctypes.c_int,
+      ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     ("mnl_plan_fill_f32", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("mnl_plan_counts", ctypes.c_int, [_vp, _vp, _vp]),
     ("mnl_plan_offsets", _vp, [_vp]),
