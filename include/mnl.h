@@ -51,6 +51,20 @@ enum {
    backend never changes the output. */
 enum { MNL_BACKEND_AUTO = -1, MNL_BACKEND_DENSE = 0, MNL_BACKEND_HASHED = 1 };
 
+/* Order of the atoms inside the dense cell grid (the reference's CellOrder,
+   src/libneighbours/cell_list.hh:25; neighbour_list.hh:98-104,140,
+   neighbour_list_gpu.hh:34,63). The reference's Morton build sorts the atoms
+   by their cell's Z-curve key (cell_list.cc:81-102 on the CPU with an
+   unstable std::sort, neighbour_list_gpu.cc:396-404 on the GPU with a stable
+   radix sort over the atom indices), so its cells hold their atoms in an
+   order the CPU leaves implementation-defined. Here every cell holds its
+   atoms in ascending index under both orders, the order of the reference's
+   stable GPU sort and of its Linear build: the pairs, their order and their
+   bits are the same for MNL_ORDER_LINEAR and MNL_ORDER_MORTON. The locality
+   the Morton curve is for comes from the engine's own cell layout, which
+   follows the input's atom order. */
+enum { MNL_ORDER_LINEAR = 0, MNL_ORDER_MORTON = 1 };
+
 /* Message of the last error on this thread ("" if none). */
 const char *mnl_last_error(void);
 
@@ -101,6 +115,42 @@ int mnl_neighbour_count_device(const double origin[3], const double cell[9],
                                const double *per_type_cutoff_sq,
                                int32_t ncutoffs, const int32_t *types,
                                int device_id, mnl_result **out);
+
+/*
+ * The request form of the two entry points above, field for field the
+ * reference's NeighbourListRequest (src/libneighbours/neighbour_list_gpu.hh:49-65)
+ * plus the stream and the grid backend. Replaces
+ * neighbour_list_gpu_device(const NeighbourListRequest&, NeighbourListDevice&)
+ * (:96) and neighbour_count_gpu_device (:101-102).
+ *   stream   hipStream_t the build runs on (NULL = default stream); host
+ *            inputs are copied on it, and the call returns once the result
+ *            is complete on it
+ *   order    MNL_ORDER_LINEAR or MNL_ORDER_MORTON (same output)
+ *   backend  MNL_BACKEND_* (AUTO: the reference's rule)
+ * Set unused fields to zero / NULL (backend AUTO is -1).
+ */
+typedef struct mnl_request {
+    int quantities;
+    const double *cell_origin;        /* [3] */
+    const double *cell;               /* [9] row-major, rows = lattice vectors */
+    const double *inv_cell;           /* [9] as given */
+    const int32_t *pbc;               /* [3] */
+    int32_t nat;
+    const double *positions;          /* [3*nat], host or device */
+    int positions_on_device;
+    double cutoff;
+    const double *per_atom_cutoff;    /* [nat] [host] or NULL */
+    const double *per_type_cutoff_sq; /* [ncutoffs^2] [host] or NULL */
+    int32_t ncutoffs;
+    const int32_t *types;             /* [nat] [host] or NULL */
+    int order;
+    int device_id;                    /* -1 = current device */
+    void *stream;
+    int backend;
+} mnl_request;
+
+int mnl_neighbour_list_request(const mnl_request *request, mnl_result **out);
+int mnl_neighbour_count_request(const mnl_request *request, mnl_result **out);
 
 /* Number of pairs in a result. */
 int64_t mnl_result_npairs(const mnl_result *res);
@@ -162,6 +212,13 @@ void mnl_plan_destroy(mnl_plan *plan);
 
 /* Device of a plan. */
 int mnl_plan_device(const mnl_plan *plan);
+
+/* CellOrder of the plan's later builds (MNL_ORDER_LINEAR, the default, or
+   MNL_ORDER_MORTON; the output is the same for both, see above), the
+   plan-side counterpart of the order argument of the reference's
+   neighbour_list / neighbour_matrix (neighbour_list.hh:98-104,134-140). */
+int mnl_plan_set_cell_order(mnl_plan *plan, int order);
+int mnl_plan_cell_order(const mnl_plan *plan);
 
 /*
  * Phase 1: bin the atoms, count every atom's neighbours and scan the counts

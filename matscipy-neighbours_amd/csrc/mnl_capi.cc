@@ -116,6 +116,10 @@ int check_device_available() {
 
 bool valid_quantities(int q) { return (q & ~31) == 0; }
 
+/* CellOrder (src/libneighbours/cell_list.hh:25): both orders give each cell
+   its atoms in ascending index, so the output is the same for both. */
+bool valid_order(int order) { return order == MNL_ORDER_LINEAR || order == MNL_ORDER_MORTON; }
+
 /* The search range of a system: refused when negative (a negative cutoff)
    or when the stencil has more than kMaxStencil neighbour-cell offsets. */
 const char *stencil_error(const SysGeom &g) {
@@ -165,6 +169,9 @@ struct mnl_plan {
     bool tri_all = false; /* no distance cutoff: every pair kept, no index table */
     /* batched geometry uploaded by mnl_plan_set_batch and still in place */
     bool batch_ready = false;
+    /* CellOrder of the plan's builds (mnl_plan_set_cell_order); both orders
+       give the same output */
+    int order = MNL_ORDER_LINEAR;
     /* workspaces carved from a fixed device array (the allocation-free
        build): they never grow, a build that needs more is refused */
     bool fixed = false;
@@ -781,11 +788,16 @@ int one_shot(int quantities, bool want_pairs, const double origin[3], const doub
              const double inv_cell[9], const int32_t pbc[3], int32_t nat,
              const double *positions, int positions_on_device, double cutoff,
              const double *per_atom, const double *per_type_sq, int32_t ncutoffs,
-             const int32_t *types, int device_id, mnl_result **out) {
+             const int32_t *types, int device_id, mnl_result **out,
+             hipStream_t stream = nullptr, int order = MNL_ORDER_LINEAR,
+             int backend = MNL_BACKEND_AUTO) {
     if (!out) return fail("out is NULL");
     *out = nullptr;
     if (!origin || !cell || !inv_cell || !pbc) return fail("geometry pointer is NULL");
     if (!valid_quantities(quantities)) return fail("Unsupported quantity specified.");
+    if (!valid_order(order)) return fail("unknown cell order (MNL_ORDER_LINEAR or _MORTON)");
+    if (backend < MNL_BACKEND_AUTO || backend > MNL_BACKEND_HASHED)
+        return fail("unknown cell-grid backend");
     if (nat > 0 && !positions) return fail("positions is NULL");
     if (per_type_sq && ncutoffs < 0) return fail("negative ncutoffs");
     if (check_device_available() != MNL_SUCCESS) return MNL_ERROR;
@@ -797,30 +809,37 @@ int one_shot(int quantities, bool want_pairs, const double origin[3], const doub
     std::lock_guard<std::mutex> lock(g_plans_mutex);
     mnl_plan *p = shared_plan(dev);
     if (!p) return MNL_ERROR;
-    hipStream_t stream = nullptr;
     const size_t n = static_cast<size_t>(std::max(nat, 1));
 
+    /* host inputs go through the plan's staging buffers, copied on the
+       build's stream (growing a buffer waits for the stream first) */
+    auto stage = [&](DevBuf &b, const void *src, size_t bytes) -> int {
+        if (bytes > b.cap) {
+            MNL_HIP(hipStreamSynchronize(stream));
+            MNL_HIP(b.ensure(bytes));
+        }
+        MNL_HIP(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, stream));
+        return MNL_SUCCESS;
+    };
     const double *dpos = positions;
     if (!positions_on_device && nat > 0) {
-        MNL_HIP(p->hpos.ensure(n * 3 * sizeof(double)));
-        MNL_HIP(hipMemcpy(p->hpos.ptr, positions, size_t(nat) * 3 * sizeof(double),
-                          hipMemcpyHostToDevice));
+        if (stage(p->hpos, positions, size_t(nat) * 3 * sizeof(double)) != MNL_SUCCESS)
+            return MNL_ERROR;
         dpos = p->hpos.as<double>();
     }
     const double *dper = nullptr;
     if (per_atom && nat > 0) {
-        MNL_HIP(p->hper_atom.ensure(n * sizeof(double)));
-        MNL_HIP(hipMemcpy(p->hper_atom.ptr, per_atom, size_t(nat) * sizeof(double),
-                          hipMemcpyHostToDevice));
+        if (stage(p->hper_atom, per_atom, size_t(nat) * sizeof(double)) != MNL_SUCCESS)
+            return MNL_ERROR;
         dper = p->hper_atom.as<double>();
     }
     const int32_t *dtypes = nullptr;
     if (types && nat > 0) {
-        MNL_HIP(p->htypes.ensure(n * sizeof(int32_t)));
-        MNL_HIP(hipMemcpy(p->htypes.ptr, types, size_t(nat) * sizeof(int32_t),
-                          hipMemcpyHostToDevice));
+        if (stage(p->htypes, types, size_t(nat) * sizeof(int32_t)) != MNL_SUCCESS)
+            return MNL_ERROR;
         dtypes = p->htypes.as<int32_t>();
     }
+    (void)n;
 
     auto *res = new mnl_result();
     res->device = dev;
@@ -829,7 +848,7 @@ int one_shot(int quantities, bool want_pairs, const double origin[3], const doub
 
     int64_t np = 0;
     if (enqueue_count(p, stream, origin, cell, inv_cell, pbc, nat, dpos, cutoff, dper,
-                      per_type_sq, ncutoffs, dtypes, MNL_BACKEND_AUTO) != MNL_SUCCESS ||
+                      per_type_sq, ncutoffs, dtypes, backend) != MNL_SUCCESS ||
         read_total(p, stream, &np) != MNL_SUCCESS) {
         delete res;
         return MNL_ERROR;
@@ -1049,6 +1068,32 @@ int mnl_neighbour_count_device(const double origin[3], const double cell[9],
                     positions_on_device, cutoff, per_atom_cutoff, per_type_cutoff_sq,
                     ncutoffs, types, device_id, out);
 }
+
+int mnl_neighbour_list_request(const mnl_request *rq, mnl_result **out) {
+    if (!rq) return fail("request is NULL");
+    return one_shot(rq->quantities, true, rq->cell_origin, rq->cell, rq->inv_cell, rq->pbc,
+                    rq->nat, rq->positions, rq->positions_on_device, rq->cutoff,
+                    rq->per_atom_cutoff, rq->per_type_cutoff_sq, rq->ncutoffs, rq->types,
+                    rq->device_id, out, static_cast<hipStream_t>(rq->stream), rq->order,
+                    rq->backend);
+}
+
+int mnl_neighbour_count_request(const mnl_request *rq, mnl_result **out) {
+    if (!rq) return fail("request is NULL");
+    return one_shot(0, false, rq->cell_origin, rq->cell, rq->inv_cell, rq->pbc, rq->nat,
+                    rq->positions, rq->positions_on_device, rq->cutoff, rq->per_atom_cutoff,
+                    rq->per_type_cutoff_sq, rq->ncutoffs, rq->types, rq->device_id, out,
+                    static_cast<hipStream_t>(rq->stream), rq->order, rq->backend);
+}
+
+int mnl_plan_set_cell_order(mnl_plan *p, int order) {
+    if (!p) return fail("plan is NULL");
+    if (!valid_order(order)) return fail("unknown cell order (MNL_ORDER_LINEAR or _MORTON)");
+    p->order = order;
+    return MNL_SUCCESS;
+}
+
+int mnl_plan_cell_order(const mnl_plan *p) { return p ? p->order : -1; }
 
 int64_t mnl_result_npairs(const mnl_result *r) { return r ? r->npairs : 0; }
 

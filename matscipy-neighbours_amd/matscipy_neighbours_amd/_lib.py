@@ -15,11 +15,26 @@ LIB_PATH = os.path.join(_HERE, "libmnl.so")
 
 QUANTITY_BITS = {"i": 1, "j": 2, "D": 4, "d": 8, "S": 16}
 BACKEND_AUTO, BACKEND_DENSE, BACKEND_HASHED = -1, 0, 1
+ORDER_LINEAR, ORDER_MORTON = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 _lp = ctypes.POINTER(ctypes.c_int64)
 _vp = ctypes.c_void_p
+
+
+
+class Request(ctypes.Structure):
+    """struct mnl_request: the reference's NeighbourListRequest
+    (src/libneighbours/neighbour_list_gpu.hh:49-65) plus stream and backend."""
+    _fields_ = [("quantities", ctypes.c_int), ("cell_origin", _vp), ("cell", _vp),
+                ("inv_cell", _vp), ("pbc", _vp), ("nat", ctypes.c_int32),
+                ("positions", _vp), ("positions_on_device", ctypes.c_int),
+                ("cutoff", ctypes.c_double), ("per_atom_cutoff", _vp),
+                ("per_type_cutoff_sq", _vp), ("ncutoffs", ctypes.c_int32), ("types", _vp),
+                ("order", ctypes.c_int), ("device_id", ctypes.c_int), ("stream", _vp),
+                ("backend", ctypes.c_int)]
+
 
 # (name, restype, argtypes) of every entry point declared in include/mnl.h.
 SIGNATURES = [
@@ -33,6 +48,12 @@ SIGNATURES = [
     ("mnl_neighbour_count_device", ctypes.c_int,
      [_vp, _vp, _vp, _vp, ctypes.c_int32, _vp, ctypes.c_int, ctypes.c_double,
       _vp, _vp, ctypes.c_int32, _vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    ("mnl_neighbour_list_request", ctypes.c_int,
+     [ctypes.POINTER(Request), ctypes.POINTER(_vp)]),
+    ("mnl_neighbour_count_request", ctypes.c_int,
+     [ctypes.POINTER(Request), ctypes.POINTER(_vp)]),
+    ("mnl_plan_set_cell_order", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("mnl_plan_cell_order", ctypes.c_int, [_vp]),
     ("mnl_result_npairs", ctypes.c_int64, [_vp]),
     ("mnl_result_nat", ctypes.c_int32, [_vp]),
     ("mnl_result_device", ctypes.c_int, [_vp]),

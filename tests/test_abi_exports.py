@@ -89,3 +89,41 @@ def test_result_cache_queries_without_device():
     assert lib.mnl_cached_bytes(-1) == 0 and lib.mnl_cached_bytes(0) == 0
     lib.mnl_release_cached_memory()
     assert lib.mnl_cached_bytes(-1) == 0
+
+
+def test_request_struct_layout_matches_header(tmp_path):
+    """The ctypes mirror of struct mnl_request has the C layout: a small C
+    program prints the size and field offsets the compiler gives."""
+    import shutil
+    import subprocess
+
+    from matscipy_neighbours_amd import _lib
+    cc = shutil.which("cc") or shutil.which("gcc")
+    if cc is None:
+        pytest.skip("no C compiler to read the header's layout")
+    fields = [f[0] for f in _lib.Request._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"mnl.h\"\nint main(void){\n"
+                   + 'printf("%zu\\n", sizeof(mnl_request));\n'
+                   + "".join(f'printf("%zu\\n", offsetof(mnl_request, {f}));\n' for f in fields)
+                   + "return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)],
+                   check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert got[0] == ctypes.sizeof(_lib.Request)
+    assert got[1:] == [getattr(_lib.Request, f).offset for f in fields]
+
+
+def test_request_and_order_arguments_without_device():
+    """Argument errors of the request entry points and the order setter are
+    reported without touching a device."""
+    from matscipy_neighbours_amd import _lib
+    lib = _lib.load()
+    out = ctypes.c_void_p()
+    assert lib.mnl_neighbour_list_request(None, ctypes.byref(out)) == -1
+    assert b"request is NULL" in lib.mnl_last_error()
+    assert lib.mnl_plan_set_cell_order(None, 1) == -1
+    assert b"plan is NULL" in lib.mnl_last_error()
+    assert lib.mnl_plan_cell_order(None) == -1
