@@ -97,35 +97,70 @@ def _log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, cell, inv, quantities, reps):
-    """Time the reference CPU core (or the oracle restatement when the
-    compiled reference is absent) on the same input."""
+def host_cpu():
+    """The host the CPU baseline ran on: CPU model, logical CPUs, and the CPUs
+    this process may use (the box's share)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return dict(cpu_model=model, nproc=os.cpu_count() or 1, usable_cpus=usable)
+
+
+def cpu_baseline(w, cell, inv, quantities, reps, single_reps=3):
+    """Time the reference CPU core, compiled from its sources into
+    oracle/_ref, on the same input: `reps` calls after one warm-up with
+    OMP_NUM_THREADS threads (the usable CPUs when unset), then `single_reps`
+    calls after one warm-up on one thread. Without the compiled reference
+    the oracle's restatement is timed instead, and the result says so
+    (kind "port", reference_missing) besides a warning on stderr."""
     import numpy as np
 
     import oracle
-    import ctypes  # noqa: F401
+    host = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", host["usable_cpus"]))
+    args = (quantities, np.zeros(3), cell, inv, w["pbc"], w["positions"], w["cutoff"])
+    extra = {}
     if oracle.ref_available():
-        secs, npairs, threads = oracle.ref_time(quantities, np.zeros(3), cell, inv,
-                                                w["pbc"], w["positions"], w["cutoff"],
-                                                reps + 1)
+        secs, npairs, threads = oracle.ref_time(*args, reps + 1, threads=threads)
+        secs1, _, _ = oracle.ref_time(*args, single_reps + 1, threads=1)
+        oracle.ref_time(*args, 1, threads=threads)  # back to the parallel setting
         kind = "reference"
         what = "reference CPU core compiled from its sources (OpenMP, -O3)"
     else:
-        t = []
-        for _ in range(reps + 1):
-            t0 = time.perf_counter()
-            out = oracle.neighbour_list(quantities, np.zeros(3), cell, inv, w["pbc"],
-                                        w["positions"], w["cutoff"])
-            t.append(time.perf_counter() - t0)
-        secs = np.array(t)
-        npairs = len(out["i"])
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        _log("WARNING: oracle/_ref/libref_neighbours.so is missing: the CPU baseline times "
+             "the oracle restatement, not the compiled reference")
+        def run(nrep):
+            t = []
+            out = None
+            for _ in range(nrep):
+                t0 = time.perf_counter()
+                out = oracle.neighbour_list(*args)
+                t.append(time.perf_counter() - t0)
+            return np.array(t), len(out["i"])
+        secs, npairs = run(reps + 1)
+        secs1 = None
         kind = "port"
         what = "oracle restatement of the reference CPU path (OpenMP, -O3)"
+        extra["reference_missing"] = "oracle/_ref/libref_neighbours.so not built in this tree"
     med = float(np.median(secs[1:]))
-    return dict(value=npairs / med, unit="pairs/s", cores=int(threads), kind=kind,
-                sample=f"full workload, {reps} timed calls after 1 warm-up, median "
-                       f"{med * 1e3:.1f} ms/call; {what}")
+    res = dict(value=npairs / med, unit="pairs/s", cores=int(threads), kind=kind,
+               sample=f"full workload, {reps} timed calls after 1 warm-up on {threads} "
+                      f"threads, median {med * 1e3:.1f} ms/call; {what}", **host, **extra)
+    if secs1 is not None:
+        med1 = float(np.median(secs1[1:]))
+        res["single_thread"] = dict(value=npairs / med1, unit="pairs/s", ms_per_call=med1 * 1e3,
+                                    calls=single_reps)
+    return res
 
 
 def batched_leg(lib, _lib, torch, dev, steps, warmup, tune=True):

@@ -16,7 +16,8 @@ def neighbor_list(quantities, atoms, cutoff, *, self_interaction=False):
     differently."""
     if self_interaction:
         raise NotImplementedError(
-            "the matscipy-neighbours-amd backend does not support self_interaction=True")
+            "self_interaction=True is not available: matscipy-neighbours-amd never "
+            "lists an atom as its own neighbour at zero shift")
     from .neighbours import neighbour_list as _neighbour_list
 
     return _neighbour_list(quantities, atoms, cutoff)

@@ -62,29 +62,34 @@ def _torch():
 
 
 def mic(dr, cell, pbc=None):
-    """Apply the minimum image convention to distance vectors ``dr``
-    (neighbours.py:39-62 of the reference)."""
-    dr = np.asarray(dr, dtype=float)
-    cell = np.asarray(cell, dtype=float)
-    rec = np.linalg.inv(cell)
+    """Minimum-image form of distance vectors ``dr`` (n x 3) in ``cell``
+    (lattice vectors as rows), as language_bindings/python/
+    matscipy_neighbours/neighbours.py:39-62 defines it: the image count is
+    the rounded (half to even) fractional coordinate of ``dr``, and ``pbc``
+    (default: every direction periodic) zeroes the Cartesian components of
+    ``dr`` that enter that fractional transform, exactly as the reference
+    scales the rows of the inverse cell."""
+    vectors = np.asarray(dr, dtype=np.float64)
+    lattice = np.asarray(cell, dtype=np.float64)
+    to_frac = np.linalg.inv(lattice)
     if pbc is not None:
-        rec = rec * np.asarray(pbc, dtype=int).reshape(3, 1)
-    offset = np.round(dr @ rec)
-    return dr - offset @ cell
+        to_frac = to_frac * np.asarray(pbc, dtype=np.int64).reshape(3, 1)
+    images = np.rint(vectors @ to_frac)
+    return vectors - images @ lattice
 
 
 class DLPackTensor:
-    """A zero-copy DLPack tensor produced by the neighbour list; implements
-    ``__dlpack__`` / ``__dlpack_device__`` (neighbours.py:65-81). Like the
-    reference it hands out one capsule, so it is single use."""
+    """One exported quantity (reference neighbours.py:65-81): an owning
+    "dltensor" capsule plus its (device type, device id). Any DLPack consumer
+    adopts it with ``from_dlpack``. The capsule can be consumed once, so the
+    object is single use, as in the reference."""
 
     __slots__ = ("_capsule", "_device")
 
     def __init__(self, capsule, device):
-        self._capsule = capsule
-        self._device = device
+        self._capsule, self._device = capsule, tuple(device)
 
-    def __dlpack__(self, *args, **kwargs):
+    def __dlpack__(self, stream=None, **_ignored):
         return self._capsule
 
     def __dlpack_device__(self):
@@ -92,78 +97,92 @@ class DLPackTensor:
 
 
 def _dlpack_device(x):
-    fn = getattr(x, "__dlpack_device__", None)
-    if fn is None:
+    """(device type, device id) that ``x`` reports through DLPack; None when
+    it has no ``__dlpack_device__`` or the call fails."""
+    query = getattr(x, "__dlpack_device__", None)
+    if not callable(query):
         return None
     try:
-        dt, did = fn()
-        return (int(dt), int(did))
-    except Exception:  # pragma: no cover - defensive
+        kind, index = query()
+    except Exception:
         return None
+    return int(kind), int(index)
 
 
 def _is_on_device(x):
-    dev = _dlpack_device(x)
-    return dev is not None and dev[0] != _DLPACK_CPU
+    """Whether ``x`` is an array in GPU memory (any DLPack device but the host)."""
+    where = _dlpack_device(x)
+    return where is not None and where[0] != _DLPACK_CPU
+
+
+# device= spellings that select the GPU (reference neighbours.py:111-128)
+_GPU_NAMES = frozenset(("cuda", "gpu", "rocm", "hip"))
 
 
 def _resolve_device(device, on_device_input):
-    """neighbours.py:111-128: None follows the input; "cpu"; "cuda"/"gpu"/
-    "rocm"/"hip"; an int id; or ("cuda", id). Returns (want_device_outputs,
-    device_id)."""
+    """(device outputs wanted, device id; -1 = current) for the ``device=``
+    argument: None follows where the positions are; "cpu" asks for host
+    outputs; a GPU name, an int id or a (name, id) pair asks for device
+    outputs. Anything else is a ValueError."""
     if device is None:
-        return (on_device_input, -1)
+        return bool(on_device_input), -1
     if isinstance(device, str):
-        d = device.lower()
-        if d == "cpu":
-            return (False, -1)
-        if d in ("cuda", "gpu", "rocm", "hip"):
-            return (True, -1)
-        raise ValueError(f"Unknown device {device!r}.")
-    if isinstance(device, int):
-        return (True, device)
-    if isinstance(device, (tuple, list)) and len(device) == 2:
-        return (True, int(device[1]))
+        name = device.lower()
+        if name == "cpu":
+            return False, -1
+        if name in _GPU_NAMES:
+            return True, -1
+    elif isinstance(device, int):
+        return True, device
+    elif isinstance(device, (tuple, list)) and len(device) == 2:
+        return True, int(device[1])
     raise ValueError(f"Unknown device {device!r}.")
 
 
 def _host_metadata(cell, pbc, numbers, cell_origin, nat, *, positions=None):
-    """neighbours.py:145-164: defaults, shrink-wrap, inv_cell = inv(cell.T)."""
+    """The host geometry of a build (reference neighbours.py:145-164):
+    origin 0, pbc all False and numbers all 1 by default; without a cell the
+    box shrink-wraps the (host) positions, origin at their minimum corner
+    unless given; inv_cell = inv(cell.T). Returns contiguous (origin, cell,
+    inv_cell, pbc bool[3], numbers int32)."""
     if cell is None:
         if positions is None:
             raise ValueError("cell must be given for device-resident positions.")
-        r = np.asarray(positions, dtype=float)
-        rmin, rmax = r.min(axis=0), r.max(axis=0)
-        cell_origin = rmin if cell_origin is None else cell_origin
-        cell = np.diag(rmax - rmin)
-    if cell_origin is None:
-        cell_origin = np.zeros(3)
-    if pbc is None:
-        pbc = np.zeros(3, dtype=bool)
-    if numbers is None:
-        numbers = np.ones(nat, dtype=np.int32)
-    cell = np.ascontiguousarray(np.asarray(cell, dtype=float))
-    cell_origin = np.ascontiguousarray(np.asarray(cell_origin, dtype=float))
-    pbc = np.ascontiguousarray(np.broadcast_to(pbc, (3,)), dtype=bool)
-    numbers = np.ascontiguousarray(np.asarray(numbers), dtype=np.int32)
-    inv_cell = np.ascontiguousarray(np.linalg.inv(cell.T))
-    return cell_origin, cell, inv_cell, pbc, numbers
+        pts = np.asarray(positions, dtype=np.float64)
+        lo = pts.min(axis=0)
+        cell = np.diag(pts.max(axis=0) - lo)
+        if cell_origin is None:
+            cell_origin = lo
+    origin = np.zeros(3) if cell_origin is None else cell_origin
+    periodic = np.zeros(3, dtype=bool) if pbc is None else pbc
+    types = np.ones(nat, dtype=np.int32) if numbers is None else numbers
+    cell = np.ascontiguousarray(cell, dtype=np.float64)
+    return (np.ascontiguousarray(origin, dtype=np.float64),
+            cell,
+            np.ascontiguousarray(np.linalg.inv(cell.T)),
+            np.ascontiguousarray(np.broadcast_to(periodic, (3,)), dtype=bool),
+            np.ascontiguousarray(types, dtype=np.int32))
+
+
+def _element_number(key):
+    """Atomic number of a cutoff-dict key: a chemical symbol or a number."""
+    return _atomic_numbers.get(key, key)
 
 
 def _resolve_cutoff(cutoff, numbers):
-    """neighbours.py:167-181: an element-pair dict becomes a symmetric
-    (maxZ+1)^2 matrix; scalars and per-atom arrays pass through."""
-    if isinstance(cutoff, dict):
-        maxnum = int(np.max(numbers))
-        matrix = np.zeros((maxnum + 1, maxnum + 1), dtype=float)
-        for (el1, el2), c in cutoff.items():
-            el1 = _atomic_numbers.get(el1, el1)
-            el2 = _atomic_numbers.get(el2, el2)
-            if el1 <= maxnum and el2 <= maxnum:
-                matrix[el1, el2] = c
-                matrix[el2, el1] = c
-        return matrix, numbers
-    return cutoff, numbers
+    """A cutoff dict keyed by element pairs becomes the symmetric per-type
+    matrix of side max(numbers) + 1 (pairs naming a larger element are
+    dropped), as in reference neighbours.py:167-181; any other cutoff passes
+    through. Returns (cutoff, numbers)."""
+    if not isinstance(cutoff, dict):
+        return cutoff, numbers
+    side = int(np.max(numbers)) + 1
+    table = np.zeros((side, side))
+    for pair, value in cutoff.items():
+        za, zb = (_element_number(e) for e in pair)
+        if za < side and zb < side:
+            table[za, zb] = table[zb, za] = value
+    return table, numbers
 
 
 def _parse_cutoff(cutoff, nat):
@@ -208,16 +227,20 @@ def _quantity_flags(quantities):
 
 
 def _gather(atoms, positions, cell, pbc, numbers, cell_origin):
-    """neighbours.py:184-197 (an ASE Atoms object or explicit arrays)."""
-    if atoms is not None:
-        if any(x is not None for x in (positions, cell, pbc, numbers, cell_origin)):
-            raise ValueError("Cannot combine an ASE Atoms object with explicit "
-                             "positions/cell/pbc/numbers/cell_origin.")
-        return (atoms.positions, np.asarray(atoms.cell), atoms.pbc,
-                np.asarray(atoms.numbers).astype(np.int32), np.zeros(3))
-    if positions is None:
-        raise ValueError("Provide either an ASE Atoms object or a positions array.")
-    return positions, cell, pbc, numbers, cell_origin
+    """(positions, cell, pbc, numbers, cell_origin) from an ASE ``Atoms`` or
+    from the explicit arguments (reference neighbours.py:184-197); device
+    positions pass through untouched. Mixing the two forms, or giving
+    neither, is a ValueError."""
+    explicit = (positions, cell, pbc, numbers, cell_origin)
+    if atoms is None:
+        if positions is None:
+            raise ValueError("Provide either an ASE Atoms object or a positions array.")
+        return explicit
+    if any(v is not None for v in explicit):
+        raise ValueError("Cannot combine an ASE Atoms object with explicit "
+                         "positions/cell/pbc/numbers/cell_origin.")
+    return (atoms.positions, np.asarray(atoms.cell), atoms.pbc,
+            np.asarray(atoms.numbers).astype(np.int32), np.zeros(3))
 
 
 def _device_positions(positions):

@@ -5,6 +5,8 @@ builds them into oracle/_ref/libref_neighbours.so):
 
     make -C oracle ref && python tests/golden/make_golden.py
 
+(`--full-sizes-only` adds or refreshes only the full-size digests.)
+
 Each case stores its inputs verbatim (positions, cell, the exact inv_cell
 bytes, pbc, cutoff specification, types) and the reference's outputs: the
 full arrays for small cases, SHA-256 digests of each array (int32 i, j, S;
@@ -137,9 +139,45 @@ def cases():
     return out
 
 
+def full_size_digests():
+    """The compiled reference's outputs at the full workload sizes, as
+    SHA-256 digests of each array (C order) plus the pair count and a digest
+    of the generated positions: the 1,048,576-atom fcc bulk (rc 5), the
+    200,000-atom per-type triclinic slab (pbc T,T,F) and the 131,072-atom
+    sheared bcc lattice (rc 6), each from its seeded generator in
+    matscipy_neighbours_amd.workloads."""
+    sys.path.insert(0, os.path.join(ROOT, "matscipy-neighbours_amd"))
+    from matscipy_neighbours_amd import workloads
+    out = {}
+    for name, w in (("fcc_1M", workloads.fcc_bulk()), ("slab_200k", workloads.triclinic_slab()),
+                    ("bcc_131k", workloads.bcc_sheared())):
+        pos = np.ascontiguousarray(w["positions"], dtype=np.float64)
+        cell = np.ascontiguousarray(w["cell"], dtype=np.float64)
+        inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+        cutoff = w["cutoff"]
+        if np.ndim(cutoff) == 2:
+            cutoff = np.ascontiguousarray(cutoff, dtype=np.float64)
+        res = oracle.ref_neighbour_list("ijdDS", np.zeros(3), cell, inv,
+                                        np.asarray(w["pbc"], bool), pos, cutoff,
+                                        types=w.get("numbers"))
+        out[name] = {"natoms": int(len(pos)), "npairs": int(len(res["i"])),
+                     "positions_digest": digest(pos),
+                     "digest": {q: digest(res[q]) for q in "ijdDS"}}
+        print(f"full size {name:10s} N={len(pos)} P={len(res['i'])}")
+    return out
+
+
 def main():
     if not oracle.ref_available():
         sys.exit("compiled reference not found: run `make -C oracle ref` first")
+    if "--full-sizes-only" in sys.argv:
+        path = os.path.join(HERE, "neighbour_list_golden.json")
+        with open(path) as f:
+            data = json.load(f)
+        data["full_sizes"] = full_size_digests()
+        with open(path, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        return
     index = {}
     arrays = {}
     for name, pos, cell, pbc, cutoff, types in cases():
@@ -216,7 +254,8 @@ def main():
     with open(os.path.join(HERE, "neighbour_list_golden.json"), "w") as f:
         json.dump({"source": "reference CPU core (src/libneighbours, CellOrder::Linear) "
                              "compiled by oracle/Makefile",
-                   "cases": index, "post": post}, f, indent=1, sort_keys=True)
+                   "cases": index, "post": post, "full_sizes": full_size_digests()},
+                  f, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":

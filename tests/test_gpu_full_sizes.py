@@ -6,6 +6,11 @@
   cutoffs 3.0/3.5/4.0 A, through both grid backends (dense and hashed);
 * the 131,072-atom sheared bcc lattice at rc 6 A.
 
+Each full list is also pinned to the compiled reference itself: the SHA-256
+digests of its i, j, d, D and S written by tests/golden/make_golden.py from
+the reference CPU core on the same seeded inputs (whose positions digest is
+checked first, so a generator change is reported as such).
+
 Plus the size-independent properties every full list has: D == r[j] - r[i] +
 S @ cell, d == |D|, the pair multiset symmetric under (i, j, S) -> (j, i, -S),
 i sorted."""
@@ -15,6 +20,7 @@ import pytest
 
 import oracle
 from conftest import assert_same_pairs
+from test_oracle import INDEX, digest
 
 pytestmark = pytest.mark.gpu
 
@@ -42,6 +48,16 @@ def _oracle(w, cell, inv):
                                  types=w.get("numbers"))
 
 
+def _reference_digests(name, w, res):
+    """Bit-exactness against the compiled reference's full-size outputs."""
+    entry = INDEX["full_sizes"][name]
+    assert digest(np.ascontiguousarray(w["positions"], dtype=np.float64)) == \
+        entry["positions_digest"], f"{name}: the seeded generator no longer gives the same input"
+    assert len(res["i"]) == entry["npairs"], name
+    for q in "ijdDS":
+        assert digest(res[q]) == entry["digest"][q], (name, q)
+
+
 def _properties(w, res, cell):
     i, j, D, d, S = res["i"], res["j"], res["D"], res["d"], res["S"]
     r = w["positions"]
@@ -63,6 +79,7 @@ def test_full_size_bitwise(gpu, name):
     w = workloads.fcc_bulk() if name == "fcc_1M" else workloads.bcc_sheared()
     print(f"{name}: {len(w['positions'])} atoms", flush=True)
     res, cell, inv = _engine(w)
+    _reference_digests(name, w, res)
     ref = _oracle(w, cell, inv)
     assert_same_pairs(res, ref, msg=name)
     _properties(w, res, cell)
@@ -75,6 +92,7 @@ def test_slab_per_type_bitwise(gpu, backend):
     w = workloads.triclinic_slab()
     print(f"slab: {len(w['positions'])} atoms, backend {backend}", flush=True)
     res, cell, inv = _engine(w, backend=backend)
+    _reference_digests("slab_200k", w, res)
     ref = _oracle(w, cell, inv)
     assert_same_pairs(res, ref, msg=f"slab backend {backend}")
     _properties(w, res, cell)

@@ -271,3 +271,23 @@ def test_oracle_matrix_matches_golden():
         assert np.array_equal(bits(dist), bits(ARR[f"{key}/dist"])), key
         assert np.array_equal(cnt, ARR[f"{key}/count"]), key
         assert ovf == c["overflow"], key
+
+
+@pytest.mark.parametrize("name", ["fcc_1M", "slab_200k", "bcc_131k"])
+def test_oracle_full_sizes_match_reference_digests(name):
+    """The restatement at the full workload sizes against the compiled
+    reference's SHA-256 digests (tests/golden/make_golden.py): the fcc bulk
+    of 1,048,576 atoms, the per-type triclinic slab and the sheared bcc."""
+    from matscipy_neighbours_amd import workloads
+    w = {"fcc_1M": workloads.fcc_bulk, "slab_200k": workloads.triclinic_slab,
+         "bcc_131k": workloads.bcc_sheared}[name]()
+    entry = INDEX["full_sizes"][name]
+    pos = np.ascontiguousarray(w["positions"], dtype=np.float64)
+    assert digest(pos) == entry["positions_digest"]
+    cell = np.ascontiguousarray(w["cell"], dtype=np.float64)
+    inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+    res = oracle.neighbour_list("ijdDS", np.zeros(3), cell, inv, np.asarray(w["pbc"], bool),
+                                pos, w["cutoff"], types=w.get("numbers"))
+    assert len(res["i"]) == entry["npairs"]
+    for q in "ijdDS":
+        assert digest(res[q]) == entry["digest"][q], (name, q)
