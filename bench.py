@@ -424,17 +424,20 @@ def leg_autograd(lib, _lib, torch, dev, steps, warmup, with_cpu):
     _lib.check(lib.mnl_plan_fill_async(step.plan.handle, step.sp, P, None, None, None, None,
                                        _lib.ptr(S), 0))
 
-    def bwd():
-        _lib.check(lib.mnl_plan_pair_backward(step.plan.handle, step.sp, P, N,
-                                              _lib.ptr(o["i"]), _lib.ptr(o["j"]), _lib.ptr(S),
-                                              _lib.ptr(o["D"]), _lib.ptr(o["d"]), _lib.ptr(gD),
-                                              _lib.ptr(gd), 0, None, _lib.ptr(gpos),
-                                              _lib.ptr(gcell)))
-    ms_b = _time_steps(torch, stream, bwd, steps, warmup)
+    def backward(det):
+        def run():
+            _lib.check(lib.mnl_plan_pair_backward(step.plan.handle, step.sp, P, N,
+                                                  _lib.ptr(o["i"]), _lib.ptr(o["j"]),
+                                                  _lib.ptr(S), _lib.ptr(o["D"]), _lib.ptr(o["d"]),
+                                                  _lib.ptr(gD), _lib.ptr(gd), 0, None, det,
+                                                  _lib.ptr(gpos), _lib.ptr(gcell)))
+        return run
+    ms_b = _time_steps(torch, stream, backward(0), steps, warmup)
+    ms_bd = _time_steps(torch, stream, backward(1), steps, warmup)
     return "bcc_131k_autograd", {
         "atoms": N, "pairs": P,
         "forward_ijdD_f64_ms": fwd["f64"][0], "forward_ijdD_f32_ms": fwd["f32"][0],
-        "backward_ms": ms_b,
+        "backward_ms": ms_b, "backward_deterministic_ms": ms_bd,
         "forward_f64_hbm_frac": (24 * N + 40 * P) / (fwd["f64"][0] * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "forward_f32_hbm_frac": (24 * N + 24 * P) / (fwd["f32"][0] * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "backward_hbm_frac": (108 * P + 24 * N) / (ms_b * 1e-3) / (HBM_PEAK_GBS * 1e9),

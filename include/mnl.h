@@ -421,16 +421,22 @@ void mnl_graph_destroy(mnl_graph *graph);
  * more outside). Both outputs are overwritten. distvec,
  * absdist and the incoming gradients are float32 when float32 != 0 (the FP32
  * output option), float64 otherwise. All pointers [device]; asynchronous.
- * The cell gradient is deterministic; grad_positions sums the j-side
- * contributions with float64 atomics (order-dependent in the last bits).
+ * The cell gradient is deterministic. With deterministic == 0,
+ * grad_positions sums the j-side contributions with float64 atomics
+ * (order-dependent in the last bits). With deterministic != 0 every sum
+ * runs in a fixed order and the result is the same bits on every call: the
+ * pairs are grouped by j with a stable radix sort, each atom adds its
+ * j-side pairs in ascending pair order and its own row in pair order (first
+ * must be sorted, as every list this library writes is; npairs < 2^31);
+ * this takes about 40 bytes of workspace per pair and is slower.
  */
 int mnl_plan_pair_backward(mnl_plan *plan, void *stream, int64_t npairs, int32_t nat,
                            const int32_t *first, const int32_t *secnd,
                            const int32_t *shift, const void *distvec,
                            const void *absdist, const void *grad_distvec,
                            const void *grad_absdist, int float32,
-                           const int32_t pbc[3], double *grad_positions,
-                           double *grad_cell);
+                           const int32_t pbc[3], int deterministic,
+                           double *grad_positions, double *grad_cell);
 
 /* ------------------------------------------------------------- autotuner
  *

@@ -141,7 +141,7 @@ struct mnl_plan {
     DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, scount, slot,
         counts, order_stats,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
-        htypes, cell_partial, tri_m, tri_cnt, tri_off, tri_vidx, tri_vstart, tri_flag, lj,
+        htypes, cell_partial, det_scratch, tri_m, tri_cnt, tri_off, tri_vidx, tri_vstart, tri_flag, lj,
         tune_out;
     int64_t *host_total = nullptr; /* pinned */
     /* state of the last count */
@@ -951,7 +951,7 @@ void mnl_plan_destroy(mnl_plan *p) {
                       &p->stype,     &p->cell_count, &p->cell_first, &p->scount, &p->slot, &p->counts, &p->order_stats, &p->offsets,
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
                       &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
-                      &p->cell_partial, &p->tri_m,   &p->tri_cnt,  &p->tri_off,
+                      &p->cell_partial, &p->det_scratch, &p->tri_m,   &p->tri_cnt,  &p->tri_off,
                       &p->tri_vidx,  &p->tri_vstart, &p->tri_flag, &p->lj, &p->tune_out};
     for (DevBuf *b : bufs) b->release();
     if (p->host_total) (void)hipHostFree(p->host_total);
@@ -1427,8 +1427,8 @@ int mnl_plan_pair_backward(mnl_plan *p, void *stream, int64_t npairs, int32_t na
                            const int32_t *first, const int32_t *secnd, const int32_t *shift,
                            const void *distvec, const void *absdist,
                            const void *grad_distvec, const void *grad_absdist,
-                           int float32, const int32_t pbc[3], double *grad_positions,
-                           double *grad_cell) {
+                           int float32, const int32_t pbc[3], int deterministic,
+                           double *grad_positions, double *grad_cell) {
     if (!p) return fail("plan is NULL");
     if (npairs < 0 || nat < 0) return fail("negative size");
     if (!grad_positions && nat > 0) return fail("grad_positions is NULL");
@@ -1442,19 +1442,29 @@ int mnl_plan_pair_backward(mnl_plan *p, void *stream, int64_t npairs, int32_t na
     DeviceScope ds(p->device);
     if (!ds.ok) return fail("cannot switch to the plan's device");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const size_t need = sizeof(double) * static_cast<size_t>(backward_scratch_doubles(nat));
-    if (need > p->cell_partial.cap) {
+    const int pbc_mask = pbc ? (pbc[0] ? 1 : 0) | (pbc[1] ? 2 : 0) | (pbc[2] ? 4 : 0) : 7;
+    DevBuf &ws = deterministic ? p->det_scratch : p->cell_partial;
+    if (deterministic && npairs >= INT32_MAX)
+        return fail("the deterministic backward takes fewer than 2^31 pairs");
+    const size_t need = deterministic
+                            ? backward_det_scratch_bytes(npairs, nat)
+                            : sizeof(double) * static_cast<size_t>(backward_scratch_doubles(nat));
+    if (need > ws.cap) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         MNL_HIP(hipStreamIsCapturing(st, &cs));
         if (cs != hipStreamCaptureStatusNone)
             return fail("mnl workspace must grow during graph capture");
         MNL_HIP(hipStreamSynchronize(st));
-        MNL_HIP(p->cell_partial.ensure(need));
+        MNL_HIP(ws.ensure(need));
     }
-    MNL_HIP(launch_pair_backward(npairs, nat, first, secnd, shift, distvec, absdist,
-                                 grad_distvec, grad_absdist, float32 != 0,
-                                 pbc ? (pbc[0] ? 1 : 0) | (pbc[1] ? 2 : 0) | (pbc[2] ? 4 : 0) : 7,
-                                 grad_positions, grad_cell, p->cell_partial.as<double>(), st));
+    if (deterministic)
+        MNL_HIP(launch_pair_backward_det(npairs, nat, first, secnd, shift, distvec, absdist,
+                                         grad_distvec, grad_absdist, float32 != 0, pbc_mask,
+                                         grad_positions, grad_cell, ws.ptr, st));
+    else
+        MNL_HIP(launch_pair_backward(npairs, nat, first, secnd, shift, distvec, absdist,
+                                     grad_distvec, grad_absdist, float32 != 0, pbc_mask,
+                                     grad_positions, grad_cell, p->cell_partial.as<double>(), st));
     return MNL_SUCCESS;
 }
 

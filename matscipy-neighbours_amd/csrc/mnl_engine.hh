@@ -245,6 +245,18 @@ int noalloc_build(int device, void *scratch, hipStream_t stream, const double or
                   int32_t *first, int32_t *secnd, void *distvec, void *absdist,
                   int32_t *shift, int float32_output, int64_t *npairs_device);
 
+/* The same backward with every sum in a fixed order (mnl_backward_det.hip):
+   the pairs grouped by j with a stable radix sort, one thread per atom
+   adding its j-side pairs and its row in pair order, the cell gradient over
+   fixed tiles. first must be sorted; npairs < 2^31. scratch holds
+   backward_det_scratch_bytes(npairs, nat) bytes (about 40 per pair). */
+size_t backward_det_scratch_bytes(int64_t npairs, int32_t nat);
+hipError_t launch_pair_backward_det(int64_t npairs, int32_t nat, const int32_t *first,
+                                    const int32_t *secnd, const int32_t *shift, const void *D,
+                                    const void *d, const void *gD, const void *gd, bool f32,
+                                    int pbc_mask, double *grad_pos, double *grad_cell,
+                                    void *scratch, hipStream_t stream);
+
 /* out[s] = offsets[atom_base[s]] for s = 0..nsys: each system's first pair. */
 hipError_t launch_system_offsets(const int64_t *offsets, const int32_t *atom_base,
                                  int32_t nsys, int64_t *out, hipStream_t stream);

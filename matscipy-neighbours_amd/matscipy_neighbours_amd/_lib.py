@@ -102,7 +102,7 @@ SIGNATURES = [
     ("mnl_graph_destroy", None, [_vp]),
     ("mnl_plan_pair_backward", ctypes.c_int,
      [_vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-      ctypes.c_int, _vp, _vp, _vp]),
+      ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]),
     ("mnl_plan_autotune", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("mnl_autotune_clear", None, []),
     ("mnl_plan_set_launch_shape", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),

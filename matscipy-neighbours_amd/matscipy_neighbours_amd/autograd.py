@@ -29,7 +29,7 @@ def _function():
 
     class NeighbourListFunction(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, positions, cell, pbc, cutoff, f32, cell_origin):
+        def forward(ctx, positions, cell, pbc, cutoff, f32, cell_origin, deterministic):
             pos = positions.detach()
             if pos.dtype != torch.float64 or not pos.is_contiguous():
                 pos = pos.to(torch.float64).contiguous()
@@ -42,6 +42,7 @@ def _function():
             i, j, D, d, S = out["i"], out["j"], out["D"], out["d"], out["S"]
             ctx.save_for_backward(i, j, S, D, d)
             ctx.pbc = np.ascontiguousarray(pbc, dtype=np.int32)
+            ctx.deterministic = bool(deterministic)
             ctx.nat = int(pos.shape[0])
             ctx.f32 = f32
             ctx.pos_dtype = positions.dtype
@@ -56,7 +57,7 @@ def _function():
             dev = D.device
             need_pos, need_cell = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
             if gD is None and gd is None:
-                return None, None, None, None, None, None
+                return None, None, None, None, None, None, None
             fdt = torch.float32 if ctx.f32 else torch.float64
             gD = None if gD is None else gD.to(fdt).contiguous()
             gd = None if gd is None else gd.to(fdt).contiguous()
@@ -70,11 +71,11 @@ def _function():
                 _lib.check(lib.mnl_plan_pair_backward(
                     plan.handle, sp, int(i.shape[0]), ctx.nat, _lib.ptr(i), _lib.ptr(j),
                     _lib.ptr(S), _lib.ptr(D), _lib.ptr(d), _lib.ptr(gD), _lib.ptr(gd),
-                    1 if ctx.f32 else 0, _lib.ptr(ctx.pbc), _lib.ptr(grad_pos),
-                    _lib.ptr(grad_cell)))
+                    1 if ctx.f32 else 0, _lib.ptr(ctx.pbc), 1 if ctx.deterministic else 0,
+                    _lib.ptr(grad_pos), _lib.ptr(grad_cell)))
             gp = grad_pos.to(ctx.pos_dtype) if need_pos else None
             gc = grad_cell.to(ctx.cell_dtype) if need_cell else None
-            return gp, gc, None, None, None, None
+            return gp, gc, None, None, None, None, None
 
     return NeighbourListFunction
 
@@ -83,13 +84,15 @@ _FN = None
 
 
 def neighbour_list_autograd(quantities, positions, cell, pbc, cutoff, *, cell_origin=None,
-                            out_dtype=None):
+                            out_dtype=None, deterministic=False):
     """Neighbour list whose D and d are differentiable with respect to
     ``positions`` ((n, 3) float64 torch tensor on the GPU) and ``cell``
     ((3, 3) torch tensor, lattice vectors as rows). ``cutoff`` is a float;
     ``quantities`` as for ``neighbour_list`` (results in that order, torch
     tensors on the positions' device). ``out_dtype=torch.float32`` gives
-    float32 D and d (pairs still decided in float64)."""
+    float32 D and d (pairs still decided in float64). ``deterministic=True``
+    makes the position gradient the same bits on every call (fixed-order
+    sums instead of float64 atomics; slower, see mnl_plan_pair_backward)."""
     global _FN
     import torch
     if not isinstance(positions, torch.Tensor) or positions.device.type != "cuda":
@@ -103,7 +106,8 @@ def neighbour_list_autograd(quantities, positions, cell, pbc, cutoff, *, cell_or
     if _FN is None:
         _FN = _function()
     pbc = np.ascontiguousarray(np.broadcast_to(np.asarray(pbc, dtype=bool), (3,)))
-    i, j, D, d, S = _FN.apply(positions, cell, pbc, cutoff, _is_f32(out_dtype), cell_origin)
+    i, j, D, d, S = _FN.apply(positions, cell, pbc, cutoff, _is_f32(out_dtype), cell_origin,
+                              bool(deterministic))
     res = dict(i=i, j=j, D=D, d=d, S=S)
     out = tuple(res[q] for q in quantities)
     return out[0] if len(out) == 1 else out

@@ -76,7 +76,7 @@ def test_argument_errors_return_status_without_device():
                            None, None, None) == -1
     assert b"plan is NULL" in lib.mnl_last_error()
     assert lib.mnl_plan_pair_backward(None, None, 0, 0, None, None, None, None, None, None,
-                                      None, 0, None, None, None) == -1
+                                      None, 0, None, 0, None, None) == -1
     assert b"plan is NULL" in lib.mnl_last_error()
 
 
