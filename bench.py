@@ -271,9 +271,10 @@ def batched_leg(lib, _lib, torch, dev, steps, warmup, tune=True):
     }
 
 
-def _plan_step_fn(lib, _lib, torch, dev, w, f32=False, flags="ijdDS"):
+def _plan_step_fn(lib, _lib, torch, dev, w, f32=False, flags="ijdDS", backend=-1):
     """A steady-state build of one workload: count (async) + fill into
-    pre-allocated outputs, as bench steps run it. Returns (step, npairs)."""
+    pre-allocated outputs, as bench steps run it (backend -1: the built-in
+    dense/hashed rule, 0 dense, 1 hashed). Returns (step, npairs)."""
     import ctypes
 
     import numpy as np
@@ -295,7 +296,8 @@ def _plan_step_fn(lib, _lib, torch, dev, w, f32=False, flags="ijdDS"):
     sp = ctypes.c_void_p(stream.cuda_stream)
     n = ctypes.c_int64(0)
     _lib.check(lib.mnl_plan_count(plan.handle, sp, P_(co), P_(cell), P_(inv), P_(pbc), N,
-                                  P_(pos), rc, None, P_(pt), nc, P_(types), -1, ctypes.byref(n)))
+                                  P_(pos), rc, None, P_(pt), nc, P_(types), backend,
+                                  ctypes.byref(n)))
     P = int(n.value)
     fdt = torch.float32 if f32 else torch.float64
     out = dict(i=torch.empty(P, dtype=torch.int32, device=dev),
@@ -307,8 +309,8 @@ def _plan_step_fn(lib, _lib, torch, dev, w, f32=False, flags="ijdDS"):
 
     def step():
         _lib.check(lib.mnl_plan_count_async(plan.handle, sp, P_(co), P_(cell), P_(inv), P_(pbc),
-                                            N, P_(pos), rc, None, P_(pt), nc, P_(types), -1,
-                                            P_(np_dev)))
+                                            N, P_(pos), rc, None, P_(pt), nc, P_(types),
+                                            backend, P_(np_dev)))
         _lib.check(lib.mnl_plan_fill_async(plan.handle, sp, P, *(P_(out[q]) if q in flags else
                                                                 None for q in "ijDdS"),
                                            1 if f32 else 0))
@@ -383,10 +385,15 @@ def leg_slab(lib, _lib, torch, dev, steps, warmup, with_cpu):
     ms = _time_steps(torch, stream, step, steps, warmup)
     N = step.N
     b = 24 * N + 4 * N + 52 * P
+    backend = "dense" if lib.mnl_plan_backend(step.plan.handle) == 0 else "hashed"
+    other = 1 if backend == "dense" else 0
+    step2, P2 = _plan_step_fn(lib, _lib, torch, dev, w, backend=other)
+    assert P2 == P
+    ms2 = _time_steps(torch, stream, step2, steps, warmup)
     return "slab_200k_pertype", {
         "atoms": N, "pairs": P, "ms_per_step": ms, "pairs_per_s": P / (ms * 1e-3),
-        "hbm_frac": b / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
-        "backend": "dense" if lib.mnl_plan_backend(step.plan.handle) == 0 else "hashed"}
+        "hbm_frac": b / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), "backend": backend,
+        f"ms_per_step_{'hashed' if other else 'dense'}_grid": ms2}
 
 
 def leg_fcc_f32(lib, _lib, torch, dev, steps, warmup, with_cpu):

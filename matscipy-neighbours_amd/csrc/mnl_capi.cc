@@ -139,7 +139,7 @@ struct mnl_plan {
     int device = 0;
     /* workspaces */
     DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, scount, slot,
-        counts, order_stats,
+        counts, order_stats, home_list, home_items,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
         htypes, cell_partial, det_scratch, tri_m, tri_cnt, tri_off, tri_vidx, tri_vstart, tri_flag, lj,
         tune_out;
@@ -209,6 +209,8 @@ PairArgs pair_args(const mnl_plan *p) {
     a.cell_first = p->cell_first.as<int32_t>();
     a.cell_count = p->cell_count.as<int32_t>();
     a.hkey = p->backend == MNL_BACKEND_HASHED ? p->hkey.as<int64_t>() : nullptr;
+    a.home_list = p->backend == MNL_BACKEND_HASHED ? p->home_list.as<int32_t>() : nullptr;
+    a.home_items = p->backend == MNL_BACKEND_HASHED ? p->home_items.as<int64_t>() : nullptr;
     a.hmask = p->hmask;
     a.spos = p->spos.as<double4>();
     a.sinfo = p->sinfo.as<int4>();
@@ -219,9 +221,11 @@ PairArgs pair_args(const mnl_plan *p) {
     /* Systems with few home cells (or slots) share each cell's atoms out
        over several waves: about 4096 waves in all, at most one per atom
        on average (fcc 1M: 97k cells, one wave each; 1000 atoms in 64
-       cells: 16 waves per cell). */
+       cells: 16 waves per cell). A hashed grid has at most one occupied
+       slot per atom. */
     {
-        const int64_t cells = std::max<int64_t>(p->home_cells, 1);
+        const int64_t cells = std::max<int64_t>(
+            p->backend == MNL_BACKEND_HASHED ? std::min(p->home_cells, p->nat) : p->home_cells, 1);
         const int64_t want = (4096 + cells - 1) / cells;
         const int64_t per_cell = (static_cast<int64_t>(p->nat) + cells - 1) / cells;
         a.split = static_cast<int32_t>(std::max<int64_t>(1, std::min(want, per_cell)));
@@ -286,6 +290,8 @@ int ensure_workspaces(mnl_plan *p, hipStream_t stream, int32_t nat, int64_t cell
         {&p->total, sizeof(int64_t)},
         {&p->hkey, hashed ? static_cast<size_t>(cells) * sizeof(int64_t) : 0},
         {&p->order_stats, 3 * sizeof(int32_t)},
+        {&p->home_list, hashed ? n * sizeof(int32_t) : 0},
+        {&p->home_items, hashed ? sizeof(int64_t) : 0},
     };
     bool grow = false;
     for (const Need &nd : needs) grow = grow || nd.bytes > nd.b->cap;
@@ -342,7 +348,18 @@ int enqueue_binning(mnl_plan *p, hipStream_t stream, const double *positions,
                                p->geo.as<SysGeom>(), p->atom_base.as<int32_t>(), p->nsys,
                                per_atom, types, p->spos.as<double4>(), p->sinfo.as<int4>(),
                                p->cut == CUT_PER_TYPE ? p->stype.as<int32_t>() : nullptr,
-                               p->scount.as<int32_t>(), p->slot.as<int32_t>(), stream));
+                               p->scount.as<int32_t>(), p->slot.as<int32_t>(),
+                               hashed ? p->ticket.as<int32_t>() : nullptr, stream));
+    if (hashed) {
+        /* the occupied slots in the order of their lowest atom: flags (in
+           ticket, free after the scatter), their scan (in tmp, free after
+           the rank/gather), the list and its length */
+        MNL_HIP(launch_scan_i32_to_i32(p->ticket.as<int32_t>(), nat, p->tmp.as<int32_t>(),
+                                       p->partial.as<int64_t>(), p->home_items.as<int64_t>(),
+                                       stream));
+        MNL_HIP(launch_home_list(p->raw4.as<int4>(), p->ticket.as<int32_t>(),
+                                 p->tmp.as<int32_t>(), nat, p->home_list.as<int32_t>(), stream));
+    }
     return MNL_SUCCESS;
 }
 
@@ -556,6 +573,8 @@ constexpr size_t noalloc_sizes[] = {
     sizeof(int64_t),                                  /* total */
     size_t(kNoallocMaxCells) * sizeof(int64_t),       /* hkey */
     3 * sizeof(int32_t),                              /* order_stats */
+    size_t(kNoallocMaxAtoms) * sizeof(int32_t),       /* home_list */
+    sizeof(int64_t),                                  /* home_items */
 };
 
 constexpr size_t noalloc_total() {
@@ -574,7 +593,8 @@ void carve_fixed_plan(mnl_plan *p, int device, void *scratch) {
     DevBuf *bufs[] = {&p->raw4,   &p->ticket,     &p->tmp,        &p->spos,    &p->sinfo,
                       &p->scount, &p->slot,       &p->counts,     &p->offsets, &p->cell_count,
                       &p->cell_first, &p->partial, &p->geo,       &p->atom_base,
-                      &p->cell_base,  &p->total,   &p->hkey,      &p->order_stats};
+                      &p->cell_base,  &p->total,   &p->hkey,      &p->order_stats,
+                      &p->home_list,  &p->home_items};
     static_assert(sizeof(bufs) / sizeof(bufs[0]) ==
                       sizeof(noalloc_sizes) / sizeof(noalloc_sizes[0]),
                   "one size per carved workspace");
@@ -949,6 +969,7 @@ void mnl_plan_destroy(mnl_plan *p) {
     DeviceScope ds(p->device);
     DevBuf *bufs[] = {&p->raw4,      &p->ticket,    &p->tmp,    &p->spos,     &p->sinfo,
                       &p->stype,     &p->cell_count, &p->cell_first, &p->scount, &p->slot, &p->counts, &p->order_stats, &p->offsets,
+                      &p->home_list, &p->home_items,
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
                       &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
                       &p->cell_partial, &p->det_scratch, &p->tri_m,   &p->tri_cnt,  &p->tri_off,

@@ -85,6 +85,10 @@ struct PairArgs {
     const int32_t *cell_base;  /* [nsys+1] */
     int32_t nsys;
     int32_t home_cells;        /* dense: total cells; hashed: slot capacity */
+    /* hashed: the occupied slots in order of their lowest atom index
+       (*home_items of them, on the device); NULL: cells 0..home_cells-1 */
+    const int32_t *home_list;
+    const int64_t *home_items;
     const int32_t *cell_first;
     const int32_t *cell_count;
     const int64_t *hkey;
@@ -134,7 +138,15 @@ hipError_t launch_rank_gather(const double *r, const int4 *raw4, const int32_t *
                               const int32_t *atom_base, int32_t nsys,
                               const double *per_atom, const int32_t *types,
                               double4 *spos, int4 *sinfo, int32_t *stype,
-                              int32_t *scount, int32_t *slot, hipStream_t stream);
+                              int32_t *scount, int32_t *slot, int32_t *first_flag,
+                              hipStream_t stream);
+/* home_list[pos[a]] = raw4[a].w for every atom a with flag[a] (the lowest
+   atom of its cell, flagged by the rank/gather; pos = exclusive scan of
+   the flags). */
+hipError_t launch_home_list(const int4 *raw4, const int32_t *flag, const int32_t *pos,
+                            int32_t nat, int32_t *home_list, hipStream_t stream);
+/* Waves a pair-kernel launch of these arguments takes. */
+int64_t pair_waves(const PairArgs &a);
 /* Exclusive scan of n int32 values; out gets n (+1 with the total) entries.
    partial needs scan_partial_size(n) int64 entries. total and total2
    (device, may be NULL) receive the sum. */

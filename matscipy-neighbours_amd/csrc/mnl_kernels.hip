@@ -358,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
     const int32_t *__restrict__ atom_base, int32_t nsys,
     const double *__restrict__ per_atom, const int32_t *__restrict__ types,
     double4 *__restrict__ spos, int4 *__restrict__ sinfo, int32_t *__restrict__ stype,
-    int32_t *__restrict__ scount, int32_t *__restrict__ slot) {
+    int32_t *__restrict__ scount, int32_t *__restrict__ slot, int32_t *__restrict__ first_flag) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nat) return;
     const int4 rc = raw4[a];
@@ -374,9 +374,20 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
         for (int u = 0; u < 8; u++) rank += v[u] < a ? 1 : 0;
     }
     const int s = b + rank;
+    if (first_flag) first_flag[a] = rank == 0 ? 1 : 0;
     const int raw[3] = {rc.x, rc.y, rc.z};
     place_atom(r, a, s, raw, geo[system_of(atom_base, nsys, a)], per_atom, types, spos, sinfo,
                stype, scount, slot);
+}
+
+/* The occupied cells (hash slots) in the order of their lowest atom. */
+__global__ __launch_bounds__(kBlock) void k_home_list(const int4 *__restrict__ raw4,
+                                                      const int32_t *__restrict__ flag,
+                                                      const int32_t *__restrict__ pos,
+                                                      int32_t nat, int32_t *__restrict__ home_list) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nat) return;
+    if (flag[a]) home_list[pos[a]] = raw4[a].w;
 }
 
 /* ---------------------------------------------------------------- scan */
@@ -714,6 +725,20 @@ __device__ __forceinline__ void wave_home(const PairArgs &A, int w, int &h, int 
     part = w - h * A.split;
 }
 
+/* Work items of a pair-kernel launch: (cell, part) for every home cell, or
+   for every entry of the home list (hashed grid), split ways each. The
+   launch's waves take them grid-stride: with cells there are as many waves
+   as items, with a list fewer (its length is known on the device only). */
+__device__ __forceinline__ int64_t pair_items(const PairArgs &A) {
+    const int64_t cells = A.home_list ? uniform(static_cast<int>(as_const(A.home_items)[0]))
+                                      : static_cast<int64_t>(A.home_cells);
+    return cells * A.split;
+}
+
+__device__ __forceinline__ int item_cell(const PairArgs &A, int idx) {
+    return A.home_list ? uniform(as_const(A.home_list)[idx]) : idx;
+}
+
 /* Home atoms [lo, hi) of part `part` of a cell with m atoms (split parts,
    sizes differing by at most one). */
 __device__ __forceinline__ void home_part(int m, int part, int split, int &lo, int &hi) {
@@ -1033,13 +1058,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     const lds_dptr pt_sq = stage_types<CUT>(A, s_pt);
     const int wib = uniform(threadIdx.x / kWave);
     const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
-    int h, part;
-    wave_home(A, blk * kWavesPerBlock + wib, h, part);
-    if (h >= A.home_cells) return;
-    if (HALF)
-        count_cell_half<CUT, HASHED>(A, h, part, s_lds[wib], pt_sq);
-    else
-        count_cell<CUT, HASHED>(A, h, part, s_lds[wib], pt_sq);
+    const int64_t items = pair_items(A);
+    for (int64_t v = blk * kWavesPerBlock + wib; v < items;
+         v += static_cast<int64_t>(gridDim.x) * kWavesPerBlock) {
+        int idx, part;
+        wave_home(A, static_cast<int>(v), idx, part);
+        const int h = item_cell(A, idx);
+        if (HALF)
+            count_cell_half<CUT, HASHED>(A, h, part, s_lds[wib], pt_sq);
+        else
+            count_cell<CUT, HASHED>(A, h, part, s_lds[wib], pt_sq);
+    }
 }
 
 /* ------------------------------------------------------- fused consumer */
@@ -1128,9 +1157,13 @@ __global__ __launch_bounds__(kBlock) void k_lj(PairArgs A, double eps4, double s
     const int wib = uniform(threadIdx.x / kWave);
     const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
     double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    int h, part;
-    wave_home(A, blk * kWavesPerBlock + wib, h, part);
-    if (h < A.home_cells) lj_cell<HASHED>(A, h, part, s_lds[wib], eps4, sig2, fx, fy, fz, acc);
+    const int64_t items = pair_items(A);
+    for (int64_t v = blk * kWavesPerBlock + wib; v < items;
+         v += static_cast<int64_t>(gridDim.x) * kWavesPerBlock) {
+        int idx, part;
+        wave_home(A, static_cast<int>(v), idx, part);
+        lj_cell<HASHED>(A, item_cell(A, idx), part, s_lds[wib], eps4, sig2, fx, fy, fz, acc);
+    }
 #pragma unroll
     for (int k = 0; k < 7; k++) acc[k] = wave_sum(acc[k]);
     if (lane_id() == 0) {
@@ -1393,10 +1426,13 @@ __global__ __launch_bounds__(kBlock) void k_fill(PairArgs A) {
     const lds_dptr pt_sq = stage_types<CUT>(A, s_pt);
     const int wib = uniform(threadIdx.x / kWave);
     const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
-    int h, part;
-    wave_home(A, blk * kWavesPerBlock + wib, h, part);
-    if (h >= A.home_cells) return;
-    fill_cell<MODE, CUT, HASHED, FC>(A, h, part, s_lds[wib], pt_sq);
+    const int64_t items = pair_items(A);
+    for (int64_t v = blk * kWavesPerBlock + wib; v < items;
+         v += static_cast<int64_t>(gridDim.x) * kWavesPerBlock) {
+        int idx, part;
+        wave_home(A, static_cast<int>(v), idx, part);
+        fill_cell<MODE, CUT, HASHED, FC>(A, item_cell(A, idx), part, s_lds[wib], pt_sq);
+    }
 }
 
 /* The previous binning measured, per axis, how far atoms 32 indices apart
@@ -1826,12 +1862,19 @@ hipError_t launch_rank_gather(const double *r, const int4 *raw4, const int32_t *
                               const int32_t *atom_base, int32_t nsys,
                               const double *per_atom, const int32_t *types, double4 *spos,
                               int4 *sinfo, int32_t *stype, int32_t *scount, int32_t *slot,
-                              hipStream_t stream) {
+                              int32_t *first_flag, hipStream_t stream) {
     if (nat <= 0) return hipSuccess;
     k_rank_gather<<<grid_for(nat), kBlock, 0, stream>>>(r, raw4, tmp, nat, cell_first,
                                                         cell_count, geo, atom_base, nsys,
                                                         per_atom, types, spos, sinfo, stype,
-                                                        scount, slot);
+                                                        scount, slot, first_flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_home_list(const int4 *raw4, const int32_t *flag, const int32_t *pos,
+                            int32_t nat, int32_t *home_list, hipStream_t stream) {
+    if (nat <= 0) return hipSuccess;
+    k_home_list<<<grid_for(nat), kBlock, 0, stream>>>(raw4, flag, pos, nat, home_list);
     return hipGetLastError();
 }
 
@@ -1953,10 +1996,21 @@ static void launch_pairs_h(int mode, bool hashed, const PairArgs &a, unsigned nb
     }
 }
 
+/* Waves walking a home list: at most one per listed cell and part (the
+   list holds at most min(slots, atoms) cells), and no more than a few
+   rounds of the chip's wave slots. */
+constexpr int64_t kListWaves = 16384;
+
+int64_t pair_waves(const PairArgs &a) {
+    const int64_t split = std::max(a.split, 1);
+    if (!a.home_list) return static_cast<int64_t>(a.home_cells) * split;
+    const int64_t cells = std::min<int64_t>(a.home_cells, std::max(a.nat, 0));
+    return std::min(cells * split, kListWaves);
+}
+
 hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
                         hipStream_t stream) {
-    /* split waves per home cell */
-    const int64_t waves = static_cast<int64_t>(args.home_cells) * std::max(args.split, 1);
+    const int64_t waves = pair_waves(args);
     if (waves <= 0) return hipSuccess;
     const unsigned nb = static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock);
     PairArgs a = args;
@@ -2054,7 +2108,7 @@ hipError_t launch_pair_backward(int64_t npairs, int32_t nat, const int32_t *firs
 constexpr int kLjReduceBlocks = 64;
 
 int64_t lj_scratch_doubles(const PairArgs &a) {
-    const int64_t waves = static_cast<int64_t>(a.home_cells) * std::max(a.split, 1);
+    const int64_t waves = pair_waves(a);
     const int64_t nb = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
     return 3 * static_cast<int64_t>(std::max(a.nat, 1)) + 10 * std::max<int64_t>(nb, 1) +
            10 * kLjReduceBlocks + 10;
@@ -2064,7 +2118,7 @@ hipError_t launch_lj(const PairArgs &args, bool hashed, double epsilon, double s
                      const int32_t *slot, double *scratch, double *forces, double *energy,
                      double *virial, hipStream_t stream) {
     const int32_t nat = args.nat;
-    const int64_t waves = static_cast<int64_t>(args.home_cells) * std::max(args.split, 1);
+    const int64_t waves = pair_waves(args);
     const int64_t nb = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
     double *fx = scratch, *fy = scratch + nat, *fz = scratch + 2 * int64_t(nat);
     double *partial = scratch + 3 * int64_t(nat);
