@@ -725,10 +725,11 @@ __device__ __forceinline__ void wave_home(const PairArgs &A, int w, int &h, int 
     part = w - h * A.split;
 }
 
-/* Work items of a pair-kernel launch: (cell, part) for every home cell, or
-   for every entry of the home list (hashed grid), split ways each. The
-   launch's waves take them grid-stride: with cells there are as many waves
-   as items, with a list fewer (its length is known on the device only). */
+/* Work items of a hashed-grid pair-kernel launch: (cell, part) for every
+   entry of the home list, split ways each, taken grid-stride by the
+   launch's waves (the list's length is known on the device only). The
+   dense grid runs one wave per (cell, part) with no loop: the loop costs
+   the dense fill about 10%. */
 __device__ __forceinline__ int64_t pair_items(const PairArgs &A) {
     const int64_t cells = A.home_list ? uniform(static_cast<int>(as_const(A.home_items)[0]))
                                       : static_cast<int64_t>(A.home_cells);
@@ -1058,6 +1059,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     const lds_dptr pt_sq = stage_types<CUT>(A, s_pt);
     const int wib = uniform(threadIdx.x / kWave);
     const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
+    if (!HASHED) {
+        int h, part;
+        wave_home(A, blk * kWavesPerBlock + wib, h, part);
+        if (h >= A.home_cells) return;
+        if (HALF)
+            count_cell_half<CUT, HASHED>(A, h, part, s_lds[wib], pt_sq);
+        else
+            count_cell<CUT, HASHED>(A, h, part, s_lds[wib], pt_sq);
+        return;
+    }
     const int64_t items = pair_items(A);
     for (int64_t v = blk * kWavesPerBlock + wib; v < items;
          v += static_cast<int64_t>(gridDim.x) * kWavesPerBlock) {
@@ -1157,12 +1168,18 @@ __global__ __launch_bounds__(kBlock) void k_lj(PairArgs A, double eps4, double s
     const int wib = uniform(threadIdx.x / kWave);
     const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
     double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const int64_t items = pair_items(A);
-    for (int64_t v = blk * kWavesPerBlock + wib; v < items;
-         v += static_cast<int64_t>(gridDim.x) * kWavesPerBlock) {
-        int idx, part;
-        wave_home(A, static_cast<int>(v), idx, part);
-        lj_cell<HASHED>(A, item_cell(A, idx), part, s_lds[wib], eps4, sig2, fx, fy, fz, acc);
+    if (!HASHED) {
+        int h, part;
+        wave_home(A, blk * kWavesPerBlock + wib, h, part);
+        if (h < A.home_cells) lj_cell<HASHED>(A, h, part, s_lds[wib], eps4, sig2, fx, fy, fz, acc);
+    } else {
+        const int64_t items = pair_items(A);
+        for (int64_t v = blk * kWavesPerBlock + wib; v < items;
+             v += static_cast<int64_t>(gridDim.x) * kWavesPerBlock) {
+            int idx, part;
+            wave_home(A, static_cast<int>(v), idx, part);
+            lj_cell<HASHED>(A, item_cell(A, idx), part, s_lds[wib], eps4, sig2, fx, fy, fz, acc);
+        }
     }
 #pragma unroll
     for (int k = 0; k < 7; k++) acc[k] = wave_sum(acc[k]);
@@ -1426,6 +1443,13 @@ __global__ __launch_bounds__(kBlock) void k_fill(PairArgs A) {
     const lds_dptr pt_sq = stage_types<CUT>(A, s_pt);
     const int wib = uniform(threadIdx.x / kWave);
     const int blk = xcd_remap(blockIdx.x, A.xcd_blocks);
+    if (!HASHED) {
+        int h, part;
+        wave_home(A, blk * kWavesPerBlock + wib, h, part);
+        if (h >= A.home_cells) return;
+        fill_cell<MODE, CUT, HASHED, FC>(A, h, part, s_lds[wib], pt_sq);
+        return;
+    }
     const int64_t items = pair_items(A);
     for (int64_t v = blk * kWavesPerBlock + wib; v < items;
          v += static_cast<int64_t>(gridDim.x) * kWavesPerBlock) {
