@@ -68,7 +68,9 @@ def _counter_means(path):
 
 def traffic_summary(fetch_csv, write_csv, cfetch_csv, cwrite_csv, commands):
     """HBM bytes per fill launch from FETCH_SIZE / WRITE_SIZE runs, each scaled
-    by what the same counter reads for a plain 1 GiB copy (--calibrate-copy)."""
+    by what the same counter reads for a plain 1 GiB copy (--calibrate-copy).
+    The summary keeps one entry per fill kernel (window size) measured on the
+    current kernel source; entries of an older source are dropped."""
     copy_bytes = float(1 << 30)
     ff = copy_bytes / (_counter_means(cfetch_csv)["__amd_rocclr_copyBuffer"] * 1024)
     wf = copy_bytes / (_counter_means(cwrite_csv)["__amd_rocclr_copyBuffer"] * 1024)
@@ -76,21 +78,45 @@ def traffic_summary(fetch_csv, write_csv, cfetch_csv, cwrite_csv, commands):
     kernel = next(k for k in fetches if k.startswith(FILL_PREFIX))
     fetch = fetches[kernel] * 1024
     write = _counter_means(write_csv)[kernel] * 1024
-    out = {
-        "kernel": kernel,
-        "workload": "fcc 64^3 cells (1048576 atoms), rc 5.0 A, ijdDS f64",
+    entry = {
         "fetch_size_bytes_raw": fetch, "write_size_bytes_raw": write,
         "copy_calibration": {"fetch_factor": ff, "write_factor": wf,
                              "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes},
         "bytes_read_per_launch": fetch * ff, "bytes_written_per_launch": write * wf,
         "bytes_per_launch": fetch * ff + write * wf,
-        "kernel_source_sha256": _source_sha(),
-        "commands": commands,
     }
+    out = {"workload": "fcc 64^3 cells (1048576 atoms), rc 5.0 A, ijdDS f64",
+           "kernel_source_sha256": _source_sha(), "kernels": {}, "commands": commands}
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            old = json.load(f)
+        if old.get("kernel_source_sha256") == out["kernel_source_sha256"]:
+            out["kernels"].update(old.get("kernels", {}))
+    out["kernels"][kernel] = entry
     with open(TRAFFIC_FILE, "w") as f:
         json.dump(out, f, indent=1)
         f.write("\n")
     print(json.dumps(out, indent=1))
+
+
+def copy_rate(torch, stream, reps=5):
+    """Measured device-to-device copy rate (GB/s, bytes read plus bytes
+    written) of a 1 GiB buffer: the best of a few copies, HIP events on the
+    stream the copies run on."""
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    best = float("inf")
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        b.copy_(a)
+        e1.record(stream)
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del a, b
+    return 2.0 * (1 << 30) / (best * 1e-3) / 1e9
 
 
 def _log(msg):
@@ -710,6 +736,9 @@ def main():
     ap.add_argument("--leg", choices=["batch"] + list(OTHER_LEGS),
                     help="run only this secondary workload (for profiling it on its own) "
                          "and print its result as one JSON line")
+    ap.add_argument("--chunks", type=int, choices=[4, 5],
+                    help="run the fcc fill with this candidate window (split 1) instead of "
+                         "the autotuned one (profiling runs of either fill kernel)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (to scale HBM "
                          "traffic counters) and exit")
@@ -721,9 +750,9 @@ def main():
     if args.traffic_from:
         traffic_summary(*args.traffic_from, commands=[
             "rocprofv3 --pmc FETCH_SIZE -f csv -d <dir> -o run -- python3 bench.py --steps 3 "
-            "--warmup 1 --no-cpu --no-batch --no-extra --no-autotune",
+            "--warmup 1 --no-cpu --no-batch --no-extra --chunks <4 or 5>",
             "rocprofv3 --pmc WRITE_SIZE -f csv -d <dir> -o run -- python3 bench.py --steps 3 "
-            "--warmup 1 --no-cpu --no-batch --no-extra --no-autotune",
+            "--warmup 1 --no-cpu --no-batch --no-extra --chunks <4 or 5>",
             "rocprofv3 --pmc FETCH_SIZE -f csv -d <dir> -o run -- python3 bench.py "
             "--calibrate-copy --steps 3",
             "rocprofv3 --pmc WRITE_SIZE -f csv -d <dir> -o run -- python3 bench.py "
@@ -790,7 +819,10 @@ def main():
                                   ctypes.byref(npairs)))
     P = int(npairs.value)
     _log(f"N={N} pairs={P}")
-    if args.no_autotune:
+    if args.chunks:
+        _lib.check(lib.mnl_plan_set_launch_shape(plan.handle, 1, args.chunks))
+        shape = {"source": "forced", "split": 1, "chunks": args.chunks}
+    elif args.no_autotune:
         # the engine's built-in window: 5 chunks when a home cell's candidates
         # (27 neighbour cells at the mean density) fit 4 < c/64 <= 5
         ncells = int(np.prod(np.floor(np.diag(cell) / w["cutoff"])))
@@ -847,6 +879,8 @@ def main():
     pipe_bytes = 24 * N + 52 * P
     fill_bytes = 52 * P + 56 * N
     achieved = fill_bytes / (fill_ms * 1e-3) / 1e9
+    copy_gbs = copy_rate(torch, stream)
+    _log(f"device-to-device copy: {copy_gbs:.0f} GB/s (bytes read + written)")
     # HBM traffic of the fill from the committed counter summary, valid only
     # while the kernel source is the one it was measured on.
     traffic = None
@@ -855,11 +889,10 @@ def main():
             tf = json.load(f)
         if tf.get("kernel_source_sha256") != _source_sha():
             _log("traffic summary is stale (kernel source changed): traffic = null")
-        elif tf.get("kernel") != fill_kernel:
-            _log(f"traffic summary is for {tf.get('kernel')}, this run used {fill_kernel}: "
-                 "traffic = null")
+        elif fill_kernel not in tf.get("kernels", {}):
+            _log(f"traffic summary has no entry for {fill_kernel}: traffic = null")
         else:
-            traffic = tf["bytes_per_launch"]
+            traffic = tf["kernels"][fill_kernel]["bytes_per_launch"]
 
     batched = None
     if not args.no_batch:
@@ -911,6 +944,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "fill_ms": fill_ms,
+            "copy_measured": copy_gbs,
+            "frac_of_copy": achieved / copy_gbs,
         },
         "launch_shape": shape,
         "cpu_baseline": cpu,
