@@ -139,7 +139,7 @@ struct mnl_plan {
     int device = 0;
     /* workspaces */
     DevBuf raw4, ticket, tmp, spos, sinfo, stype, cell_count, cell_first, scount, slot,
-        counts, order_stats, home_list, home_items,
+        counts, order_stats, home_list, home_items, item_base,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
         htypes, cell_partial, det_scratch, tri_m, tri_cnt, tri_off, tri_vidx, tri_vstart, tri_flag, lj,
         tune_out;
@@ -151,6 +151,7 @@ struct mnl_plan {
     int backend = MNL_BACKEND_DENSE;
     int cut = CUT_GLOBAL;
     int32_t home_cells = 0;
+    int32_t batch_items = 0; /* batches: waves of a pair-kernel launch */
     int64_t hmask = 0;
     int64_t npairs = 0;
     int32_t ncut = 0;
@@ -241,6 +242,12 @@ PairArgs pair_args(const mnl_plan *p) {
             a.split = it->second.split;
             a.fill_chunks = it->second.chunks;
         }
+    }
+    if (p->batch_ready) {
+        /* batches: per-system splits replace the uniform one */
+        a.item_base = p->item_base.as<int32_t>();
+        a.items = p->batch_items;
+        a.split = 1;
     }
     a.atom_base = p->atom_base.as<int32_t>();
     a.nat = p->nat;
@@ -969,7 +976,7 @@ void mnl_plan_destroy(mnl_plan *p) {
     DeviceScope ds(p->device);
     DevBuf *bufs[] = {&p->raw4,      &p->ticket,    &p->tmp,    &p->spos,     &p->sinfo,
                       &p->stype,     &p->cell_count, &p->cell_first, &p->scount, &p->slot, &p->counts, &p->order_stats, &p->offsets,
-                      &p->home_list, &p->home_items,
+                      &p->home_list, &p->home_items, &p->item_base,
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
                       &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
                       &p->cell_partial, &p->det_scratch, &p->tri_m,   &p->tri_cnt,  &p->tri_off,
@@ -1270,6 +1277,12 @@ int mnl_plan_matrix(mnl_plan *p, void *stream, const double origin[3], const dou
     return MNL_SUCCESS;
 }
 
+namespace {
+/* candidate tests a batch wave aims at (fcc 1M: about 3,100 per home cell;
+   on the 1024-system batch 8192 measured best of 1024 ... 32768) */
+constexpr double kBatchWaveTests = 8192.0;
+}  // namespace
+
 int mnl_plan_set_batch(mnl_plan *p, void *stream, int32_t nsys, const int32_t *atom_offsets,
                        const double *origins, const double *cells, const double *inv_cells,
                        const int32_t *pbcs, const double *cutoffs) {
@@ -1307,8 +1320,32 @@ int mnl_plan_set_batch(mnl_plan *p, void *stream, int32_t nsys, const int32_t *a
     const int32_t nat = atom_offsets[nsys];
     abase[static_cast<size_t>(nsys)] = nat;
     cbase[static_cast<size_t>(nsys)] = static_cast<int32_t>(ncells);
+    /* Waves per home cell, per system: a home cell costs about m * (nseg m)
+       candidate tests at m atoms per cell. A system of one cell holds all
+       its atoms there and tests 27 images of it: 50 atoms make 67,500 tests,
+       twenty times a typical cell, so such cells are shared out over several
+       waves (at most one per atom) to keep every wave under about
+       kBatchWaveTests. */
+    std::vector<int32_t> ibase(static_cast<size_t>(nsys) + 1);
+    int64_t items = 0;
+    for (int32_t s = 0; s < nsys; s++) {
+        SysGeom &g = geo[static_cast<size_t>(s)];
+        const double cells_s = std::max<double>(1.0, double(g.ncells));
+        const double m = double(g.natoms) / cells_s;
+        const double tests = m * m * double(g.nseg);
+        const int64_t want = static_cast<int64_t>(std::ceil(tests / kBatchWaveTests));
+        const int64_t cap = std::max<int64_t>(1, static_cast<int64_t>(std::ceil(m)));
+        g.split = static_cast<int32_t>(std::max<int64_t>(1, std::min(want, cap)));
+        ibase[static_cast<size_t>(s)] = static_cast<int32_t>(items);
+        items += static_cast<int64_t>(g.ncells) * g.split;
+        if (items > INT32_MAX / 2) return fail("batched work items too many");
+    }
+    ibase[static_cast<size_t>(nsys)] = static_cast<int32_t>(items);
     if (ensure_workspaces(p, st, nat, ncells, false, CUT_GLOBAL, 0, nsys) != MNL_SUCCESS)
         return MNL_ERROR;
+    MNL_HIP(p->item_base.ensure(sizeof(int32_t) * ibase.size()));
+    MNL_HIP(hipMemcpyAsync(p->item_base.ptr, ibase.data(), sizeof(int32_t) * ibase.size(),
+                           hipMemcpyHostToDevice, st));
     MNL_HIP(hipMemcpyAsync(p->geo.ptr, geo.data(), sizeof(SysGeom) * geo.size(),
                            hipMemcpyHostToDevice, st));
     MNL_HIP(hipMemcpyAsync(p->atom_base.ptr, abase.data(), sizeof(int32_t) * abase.size(),
@@ -1324,6 +1361,7 @@ int mnl_plan_set_batch(mnl_plan *p, void *stream, int32_t nsys, const int32_t *a
     p->ncut = 0;
     p->hmask = 0;
     p->home_cells = static_cast<int32_t>(ncells);
+    p->batch_items = static_cast<int32_t>(items);
     p->stream = st;
     p->batch_ready = true;
     return MNL_SUCCESS;
@@ -1558,6 +1596,7 @@ int mnl_plan_autotune(mnl_plan *p, void *stream, int32_t *split, int32_t *chunks
     std::vector<TuneShape> cand;
     for (int32_t sp : {1, 2, 4, 8}) {
         if (sp > 1 && sp > 2 * per_cell) continue;
+        if (sp > 1 && p->batch_ready) continue; /* batches split per system */
         for (int32_t ch : {4, 5}) cand.push_back(TuneShape{sp, ch});
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -42,6 +42,7 @@ struct SysGeom {
     int32_t cell_base;  /* first dense cell of the system */
     int32_t ncells;
     int32_t nseg;       /* (2nx+1)(2ny+1)(2nz+1) neighbour-cell offsets */
+    int32_t split;      /* batches: waves per home cell of this system */
     /* dense cell layout: cell = c[f] + n[f] * (c[m] + n[m] * c[s]) with
        (f, m, s) = lay; {0, 1, 2} (x fastest) unless the binning of the
        previous build found the input's atom order running along another
@@ -89,6 +90,11 @@ struct PairArgs {
        (*home_items of them, on the device); NULL: cells 0..home_cells-1 */
     const int32_t *home_list;
     const int64_t *home_items;
+    /* batches: first work item of each system ([nsys+1]; item = one of a
+       home cell's geo[s].split waves), and their total; NULL: uniform
+       split over all home cells */
+    const int32_t *item_base;
+    int32_t items;
     const int32_t *cell_first;
     const int32_t *cell_count;
     const int64_t *hkey;

@@ -87,6 +87,7 @@ bool make_sys_geom(const double origin[3], const double cell[9],
     g.ncells = nc > INT32_MAX ? INT32_MAX : static_cast<int32_t>(nc);
     const int64_t nseg = stencil_size(g.nxyz);
     g.nseg = nseg > INT32_MAX ? INT32_MAX : static_cast<int32_t>(nseg);
+    g.split = 1;
     for (int k = 0; k < 3; k++) g.lay[k] = k;
     int32_t divisors[5] = {g.n[0], g.n[1], g.n[2], 1, 1};
     if (nseg <= kMaxStencil) {

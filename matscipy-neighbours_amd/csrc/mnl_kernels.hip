@@ -646,6 +646,7 @@ struct HomeCell {
     int hsx, hsy;
     int p0, p1, p2;
     int abase, cbase;
+    int split;             /* waves sharing this cell's home atoms */
     int lf, lm, ls, nf, nm; /* dense layout: fastest, middle, slowest axis */
     double cutoff_sq;
     double b00, b01, b02, b10, b11, b12, b20, b21, b22;
@@ -714,6 +715,7 @@ __device__ __forceinline__ bool load_home(const PairArgs &A, int h, HomeCell &H)
     H.rn2 = gp->rn[2];
     H.abase = gp->atom_base;
     H.cbase = gp->cell_base;
+    H.split = A.item_base ? gp->split : A.split;
     return true;
 }
 
@@ -721,6 +723,25 @@ __device__ __forceinline__ bool load_home(const PairArgs &A, int h, HomeCell &H)
    are shared out over its split waves (each atom's row is independent), so
    that systems with few cells still give the chip thousands of waves. */
 __device__ __forceinline__ void wave_home(const PairArgs &A, int w, int &h, int &part) {
+    if (A.item_base) {
+        /* a batch: each system's cells with the system's own split, so that
+           the one-cell systems (all their atoms in one home cell, 27 images
+           of it as candidates) do not leave a few waves running long after
+           the rest */
+        if (w >= A.items) {
+            h = A.home_cells;
+            part = 0;
+            return;
+        }
+        const int s = uniform(system_of(A.item_base, A.nsys, w));
+        cptr<SysGeom> gp = as_const(A.geo) + s;
+        const int sp = uniform(gp->split);
+        const int local = w - uniform(as_const(A.item_base)[s]);
+        const int c = sp > 1 ? local / sp : local;
+        h = uniform(gp->cell_base) + c;
+        part = local - c * sp;
+        return;
+    }
     h = A.split > 1 ? w / A.split : w;
     part = w - h * A.split;
 }
@@ -917,7 +938,7 @@ __device__ __forceinline__ void count_cell(const PairArgs &A, int h, int part, S
     if (!load_home<HASHED>(A, h, H)) return;
     const int lane = lane_id();
     int lo, hi;
-    home_part(H.m, part, A.split, lo, hi);
+    home_part(H.m, part, H.split, lo, hi);
     for (int grp0 = lo; grp0 < hi; grp0 += kWave) {
         const int mg = uniform(min(kWave, hi - grp0));
         int acc = 0;
@@ -1005,7 +1026,7 @@ __device__ __forceinline__ void count_cell_half(const PairArgs &A, int h, int pa
     if (!load_home<HASHED>(A, h, H)) return;
     const int lane = lane_id();
     int lo, hi;
-    home_part(H.m, part, A.split, lo, hi);
+    home_part(H.m, part, H.split, lo, hi);
     for (int grp0 = lo; grp0 < hi; grp0 += kWave) {
         const int mg = uniform(min(kWave, hi - grp0));
         int acc = 0;
@@ -1109,7 +1130,7 @@ __device__ __forceinline__ void lj_cell(const PairArgs &A, int h, int part, SegL
     if (!load_home<HASHED>(A, h, H)) return;
     const int lane = lane_id();
     int lo, hi;
-    home_part(H.m, part, A.split, lo, hi);
+    home_part(H.m, part, H.split, lo, hi);
     for (int grp0 = lo; grp0 < hi; grp0 += kWave) {
         const int mg = uniform(min(kWave, hi - grp0));
         for (int sg = 0; sg < H.nseg; sg += kWave) {
@@ -1337,7 +1358,7 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, int part, Fi
     if (!load_home<HASHED>(A, h, H)) return;
     const int lane = lane_id();
     int lo, hi;
-    home_part(H.m, part, A.split, lo, hi);
+    home_part(H.m, part, H.split, lo, hi);
     for (int grp0 = lo; grp0 < hi; grp0 += kWave) {
         const int mg = uniform(min(kWave, hi - grp0));
         if (lane < mg) {
@@ -2026,6 +2047,7 @@ static void launch_pairs_h(int mode, bool hashed, const PairArgs &a, unsigned nb
 constexpr int64_t kListWaves = 16384;
 
 int64_t pair_waves(const PairArgs &a) {
+    if (a.item_base) return a.items;
     const int64_t split = std::max(a.split, 1);
     if (!a.home_list) return static_cast<int64_t>(a.home_cells) * split;
     const int64_t cells = std::min<int64_t>(a.home_cells, std::max(a.nat, 0));
