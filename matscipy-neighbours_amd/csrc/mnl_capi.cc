@@ -170,6 +170,7 @@ struct mnl_plan {
     bool tri_all = false; /* no distance cutoff: every pair kept, no index table */
     /* batched geometry uploaded by mnl_plan_set_batch and still in place */
     bool batch_ready = false;
+    bool batch_small = false; /* every system fits one binning block */
     /* CellOrder of the plan's builds (mnl_plan_set_cell_order); both orders
        give the same output */
     int order = MNL_ORDER_LINEAR;
@@ -330,6 +331,14 @@ int enqueue_binning(mnl_plan *p, hipStream_t stream, const double *positions,
     const bool hashed = p->backend == MNL_BACKEND_HASHED;
     const int64_t cells = p->home_cells;
     const int32_t nat = p->nat;
+    if (p->batch_ready && p->batch_small) {
+        /* one block per system writes every cell count (no clearing) */
+        MNL_HIP(launch_bin_systems(positions, p->nsys, p->geo.as<SysGeom>(),
+                                   p->cell_first.as<int32_t>(), p->cell_count.as<int32_t>(),
+                                   p->spos.as<double4>(), p->sinfo.as<int4>(),
+                                   p->scount.as<int32_t>(), p->slot.as<int32_t>(), stream));
+        return MNL_SUCCESS;
+    }
     if (!cleared) MNL_HIP(launch_fill_i32(p->cell_count.as<int32_t>(), cells, 0, stream));
     if (hashed) {
         if (!cleared) MNL_HIP(launch_fill_i64(p->hkey.as<int64_t>(), cells, -1, stream));
@@ -1328,6 +1337,7 @@ int mnl_plan_set_batch(mnl_plan *p, void *stream, int32_t nsys, const int32_t *a
        kBatchWaveTests. */
     std::vector<int32_t> ibase(static_cast<size_t>(nsys) + 1);
     int64_t items = 0;
+    int64_t max_atoms = 0, max_cells = 0;
     for (int32_t s = 0; s < nsys; s++) {
         SysGeom &g = geo[static_cast<size_t>(s)];
         const double cells_s = std::max<double>(1.0, double(g.ncells));
@@ -1337,6 +1347,8 @@ int mnl_plan_set_batch(mnl_plan *p, void *stream, int32_t nsys, const int32_t *a
         const int64_t cap = std::max<int64_t>(1, static_cast<int64_t>(std::ceil(m)));
         g.split = static_cast<int32_t>(std::max<int64_t>(1, std::min(want, cap)));
         ibase[static_cast<size_t>(s)] = static_cast<int32_t>(items);
+        max_atoms = std::max<int64_t>(max_atoms, g.natoms);
+        max_cells = std::max<int64_t>(max_cells, g.ncells);
         items += static_cast<int64_t>(g.ncells) * g.split;
         if (items > INT32_MAX / 2) return fail("batched work items too many");
     }
@@ -1362,6 +1374,7 @@ int mnl_plan_set_batch(mnl_plan *p, void *stream, int32_t nsys, const int32_t *a
     p->hmask = 0;
     p->home_cells = static_cast<int32_t>(ncells);
     p->batch_items = static_cast<int32_t>(items);
+    p->batch_small = batch_binning_fits(max_atoms, max_cells);
     p->stream = st;
     p->batch_ready = true;
     return MNL_SUCCESS;

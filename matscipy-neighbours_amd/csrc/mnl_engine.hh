@@ -214,6 +214,14 @@ hipError_t launch_lj(const PairArgs &args, bool hashed, double epsilon, double s
    the cell scan, scatter and rank_gather write, except the scatter's
    scratch (raw4, ticket, tmp). */
 bool small_binning_fits(int64_t nat, int64_t ncells);
+/* The binning of a batch whose systems all fit one block each (at most
+   2048 atoms and 2048 cells: batch_binning_fits), one block per system, from
+   the uploaded geometry: writes what bin_dense, the cell scan, scatter and
+   rank_gather write, except the scatter's scratch. */
+bool batch_binning_fits(int64_t max_atoms, int64_t max_cells);
+hipError_t launch_bin_systems(const double *r, int32_t nsys, const SysGeom *geo,
+                              int32_t *cell_first, int32_t *cell_count, double4 *spos,
+                              int4 *sinfo, int32_t *scount, int32_t *slot, hipStream_t stream);
 hipError_t launch_bin_small(const double *r, int32_t nat, const SysGeom &g, int32_t ncells,
                             SysGeom *geo, int32_t *atom_base, int32_t *cell_base,
                             int32_t *order_stats, const double *per_atom, const int32_t *types,

@@ -1657,6 +1657,100 @@ __global__ __launch_bounds__(kSmallThreads) void k_bin_small(
     }
 }
 
+/*
+ * The whole binning of a batch of small systems, one block per system (up
+ * to kSysAtoms atoms and kSysCells cells each): the same results as
+ * k_bin_dense, the cell scan, k_scatter and k_rank_gather over the batch.
+ * Every system's atoms land in its own cells, so its cell slices start at
+ * its atom base and the block's own scan places them. The histogram, the
+ * scan and the member lists live in LDS.
+ */
+constexpr int kSysThreads = 256;
+constexpr int kSysAtoms = 2048;
+constexpr int kSysCells = 2048;
+constexpr int kSysPer = kSysAtoms / kSysThreads;
+
+__global__ __launch_bounds__(kSysThreads) void k_bin_systems(
+    const double *__restrict__ r, const SysGeom *__restrict__ geo,
+    int32_t *__restrict__ cell_first, int32_t *__restrict__ cell_count,
+    double4 *__restrict__ spos, int4 *__restrict__ sinfo, int32_t *__restrict__ scount,
+    int32_t *__restrict__ slot) {
+    __shared__ int s_cnt[kSysCells];
+    __shared__ int s_first[kSysCells];
+    __shared__ int s_tmp[kSysAtoms];
+    __shared__ int s_w[kSysThreads / kWave];
+    __shared__ SysGeom s_g;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_g = geo[blockIdx.x];
+    __syncthreads();
+    const SysGeom &g = s_g;
+    const int nat = g.natoms, ncells = g.ncells, abase = g.atom_base, cbase = g.cell_base;
+    for (int c = tid; c < ncells; c += kSysThreads) s_cnt[c] = 0;
+    __syncthreads();
+
+    /* cells, and tickets in arrival order */
+    int raw[kSysPer][3], cell[kSysPer], tk[kSysPer];
+#pragma unroll
+    for (int u = 0; u < kSysPer; u++) {
+        const int a = tid + u * kSysThreads;
+        cell[u] = -1;
+        tk[u] = 0;
+        raw[u][0] = raw[u][1] = raw[u][2] = 0;
+        if (a < nat) {
+            raw_cell_of(r, abase + a, g, raw[u]);
+            int c[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                c[k] = g.pbc[k] ? wrap_idx(raw[u][k], g.n[k]) : clamp_idx(raw[u][k], g.n[k]);
+            cell[u] = dense_index(g, c[0], c[1], c[2]);
+            tk[u] = atomicAdd(&s_cnt[cell[u]], 1);
+        }
+    }
+    __syncthreads();
+
+    /* exclusive scan of the cell counts: kSysPer consecutive cells per
+       thread, then across the block */
+    int loc[kSysPer];
+    int sum = 0;
+#pragma unroll
+    for (int u = 0; u < kSysPer; u++) {
+        const int c = tid * kSysPer + u;
+        loc[u] = sum;
+        sum += c < ncells ? s_cnt[c] : 0;
+    }
+    const int incl = wave_incl_scan(sum);
+    if (lane_id() == kWave - 1) s_w[tid / kWave] = incl;
+    __syncthreads();
+    int pre = incl - sum;
+    for (int w = 0; w < tid / kWave; w++) pre += s_w[w];
+#pragma unroll
+    for (int u = 0; u < kSysPer; u++) {
+        const int c = tid * kSysPer + u;
+        if (c < ncells) {
+            s_first[c] = pre + loc[u];
+            cell_first[cbase + c] = abase + pre + loc[u];
+            cell_count[cbase + c] = s_cnt[c];
+        }
+    }
+    __syncthreads();
+
+    /* member lists, then the stable rank: members with a smaller index */
+#pragma unroll
+    for (int u = 0; u < kSysPer; u++)
+        if (cell[u] >= 0) s_tmp[s_first[cell[u]] + tk[u]] = tid + u * kSysThreads;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kSysPer; u++) {
+        if (cell[u] < 0) continue;
+        const int a = tid + u * kSysThreads;
+        const int b = s_first[cell[u]], m = s_cnt[cell[u]];
+        int rank = 0;
+        for (int t = 0; t < m; t++) rank += s_tmp[b + t] < a ? 1 : 0;
+        place_atom(r, abase + a, abase + b + rank, raw[u], g, nullptr, nullptr, spos, sinfo,
+                   nullptr, scount, slot);
+    }
+}
+
 /* ---------------------------------------------------------------- backward */
 
 /* Inclusive sum over the lanes of the same i-segment (pairs are i-sorted, so
@@ -2069,6 +2163,19 @@ hipError_t launch_pairs(int mode, int cut, bool hashed, const PairArgs &args,
         launch_pairs_h<CUT_PER_TYPE>(mode, hashed, a, nb, stream);
     else
         launch_pairs_h<CUT_GLOBAL>(mode, hashed, a, nb, stream);
+    return hipGetLastError();
+}
+
+bool batch_binning_fits(int64_t max_atoms, int64_t max_cells) {
+    return max_atoms <= kSysAtoms && max_cells <= kSysCells;
+}
+
+hipError_t launch_bin_systems(const double *r, int32_t nsys, const SysGeom *geo,
+                              int32_t *cell_first, int32_t *cell_count, double4 *spos,
+                              int4 *sinfo, int32_t *scount, int32_t *slot, hipStream_t stream) {
+    if (nsys <= 0) return hipSuccess;
+    k_bin_systems<<<static_cast<unsigned>(nsys), kSysThreads, 0, stream>>>(
+        r, geo, cell_first, cell_count, spos, sinfo, scount, slot);
     return hipGetLastError();
 }
 
