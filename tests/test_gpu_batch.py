@@ -106,6 +106,43 @@ def test_batch_mixed_systems(gpu):
     _check_batch(systems, (i, j, D, d, S, off))
 
 
+def _mixed_small(rng, count, big=0):
+    """Systems of 0-300 atoms (and optionally one of `big` atoms), at most
+    10 cells per direction, mixed periodicity, atoms up to 20% outside."""
+    systems = []
+    for k in range(count):
+        n = int(rng.integers(0, 300)) if k % 7 else 0
+        if big and k == count // 2:
+            n = big
+        edge = rng.uniform(3.0, 25.0)
+        cell = edge * (np.eye(3) + rng.uniform(-0.1, 0.1, (3, 3)) * (k % 2))
+        pb = rng.integers(0, 2, 3).astype(bool)
+        frac = rng.uniform(-0.2, 1.2, (n, 3))
+        systems.append(dict(positions=frac @ cell, cell=cell, pbc=pb,
+                            cutoff=float(rng.uniform(edge / 9.0, 6.0 + edge / 9.0))))
+    return systems
+
+
+@pytest.mark.parametrize("big", [0, 3000])
+def test_batch_binning_paths(gpu, big):
+    """Batches whose systems all have at most 2048 atoms and 2048 cells are
+    binned one block per system; one system of 3000 atoms sends the batch
+    through the general binning. Both give every system bit-exact pairs,
+    with clamped non-periodic directions, atoms outside the cell, empty
+    systems, and per-system wave splits (one-cell systems of many atoms)."""
+    from matscipy_neighbours_amd import neighbour_list_batch
+    rng = np.random.default_rng(31 + big)
+    systems = _mixed_small(rng, 40, big)
+    for s in systems:
+        c = np.asarray(s["cell"])
+        faces = abs(np.linalg.det(c)) / np.linalg.norm(np.cross(c[[1, 2, 0]], c[[2, 0, 1]]), axis=1)
+        assert np.prod(np.maximum(1, np.floor(faces / s["cutoff"]))) <= 2048
+    pos, cells, pbc, rc, nat = _concat(systems)
+    print(f"batch: {len(systems)} systems, {len(pos)} atoms, largest {nat.max()}", flush=True)
+    out = neighbour_list_batch("ijDdS", pos, cells, pbc, rc, nat)
+    _check_batch(systems, out)
+
+
 def test_batch_f32_output(gpu):
     import numpy as _np
     from matscipy_neighbours_amd import neighbour_list_batch, workloads
