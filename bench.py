@@ -125,7 +125,7 @@ def _log(msg):
 
 def host_cpu():
     """The host the CPU baseline ran on: CPU model, logical CPUs, and the CPUs
-    this process may use (the box's share)."""
+    this process may use (its affinity mask)."""
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
