@@ -422,6 +422,28 @@ def leg_slab(lib, _lib, torch, dev, steps, warmup, with_cpu):
         f"ms_per_step_{'hashed' if other else 'dense'}_grid": ms2}
 
 
+def leg_vacuum(lib, _lib, torch, dev, steps, warmup, with_cpu):
+    """131,072 atoms in a corner of a 20x wider periodic box: the built-in rule
+    takes the hashed grid (about 10^8 dense cells); the dense grid is timed
+    beside it."""
+    from matscipy_neighbours_amd import workloads
+    stream = torch.cuda.current_stream(dev)
+    w = workloads.vacuum_cluster()
+    step, P = _plan_step_fn(lib, _lib, torch, dev, w)
+    ms = _time_steps(torch, stream, step, steps, warmup)
+    N = step.N
+    backend = "dense" if lib.mnl_plan_backend(step.plan.handle) == 0 else "hashed"
+    other = 1 if backend == "dense" else 0
+    step2, P2 = _plan_step_fn(lib, _lib, torch, dev, w, backend=other)
+    assert P2 == P
+    ms2 = _time_steps(torch, stream, step2, steps, warmup)
+    del step2
+    return "vacuum_cluster_131k", {
+        "atoms": N, "pairs": P, "ms_per_step": ms, "pairs_per_s": P / (ms * 1e-3),
+        "hbm_frac": (24 * N + 52 * P) / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "backend": backend, f"ms_per_step_{'hashed' if other else 'dense'}_grid": ms2}
+
+
 def leg_fcc_f32(lib, _lib, torch, dev, steps, warmup, with_cpu):
     """The headline fcc workload with the float32 output option (D, d
     float32; pairs still decided in float64): 36 B per pair instead of 52."""
@@ -699,7 +721,8 @@ def leg_alloc(lib, _lib, torch, dev, steps, warmup, with_cpu):
                "including a device synchronisation, median"}
 
 
-OTHER_LEGS = {"latency": leg_latency, "slab": leg_slab, "fcc_f32": leg_fcc_f32,
+OTHER_LEGS = {"latency": leg_latency, "slab": leg_slab, "vacuum": leg_vacuum,
+              "fcc_f32": leg_fcc_f32,
               "autograd": leg_autograd, "lj": leg_lj, "alloc": leg_alloc,
               "post": leg_post}
 

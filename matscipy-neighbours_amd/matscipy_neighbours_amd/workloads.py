@@ -44,6 +44,18 @@ def triclinic_slab(n=200_000, seed=3):
                 cutoff=np.array([[3.0, 3.5], [3.5, 4.0]]))
 
 
+def vacuum_cluster(n=131_072, density=0.08, factor=20.0, cutoff=5.0, seed=6):
+    """n atoms uniform in a cube of edge (n/density)^(1/3) in the corner of a
+    periodic cubic box `factor` times wider: mostly vacuum, so the cell grid
+    (about 471^3 cells at the defaults) exceeds max(2^20, 8n) and the
+    reference's rule picks the hashed grid (the shape of the reference's
+    sparse-vacuum test, tests/test_neighbour_list_gpu.cc:116-123, at a larger n)."""
+    rng = np.random.default_rng(seed)
+    edge = (n / density) ** (1.0 / 3.0)
+    return dict(positions=rng.uniform(0.0, edge, (n, 3)), cell=factor * edge * np.eye(3),
+                pbc=np.array([True, True, True]), cutoff=float(cutoff))
+
+
 def bcc_sheared(reps=(64, 32, 32), a=2.87, sigma=0.08, shear=0.05, cutoff=6.0, seed=5):
     """bcc lattice (2 sites per cubic cell), reps cells, Gaussian jitter,
     periodic cell diag(reps*a) with an xy shear; 64x32x32 gives 131,072 atoms."""
