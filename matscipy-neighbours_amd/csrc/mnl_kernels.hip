@@ -1753,17 +1753,46 @@ __global__ __launch_bounds__(kSysThreads) void k_bin_systems(
 
 /* ---------------------------------------------------------------- backward */
 
-/* Inclusive sum over the lanes of the same i-segment (pairs are i-sorted, so
-   equal keys are contiguous within the wave). */
-__device__ __forceinline__ double seg_incl_sum(double v, int key) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const double t = __shfl_up(v, o, kWave);
-        const int tk = __shfl_up(key, o, kWave);
-        if (lane >= o && tk == key) v += t;
+/* Inclusive sums of (a, b, c) over the lanes of the same key (keys are
+   non-decreasing across the wave, so equal keys are contiguous): lane l adds
+   lane l - s's partial when their keys match, for s = 1, 2, 4, 8 inside each
+   row of 16 lanes (DPP row shifts), then the last lane of row 0 into row 1
+   and of row 2 into row 3, then lane 31 into rows 2 and 3 (DPP row
+   broadcasts). The lane moves run on the VALU rather than through LDS, and
+   the key moves once per step for all three sums. A lane without a source
+   receives 0 and no key. */
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(u & 0xffffffffu), CTRL,
+                                               ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(u >> 32), CTRL, ROWS, 0xf,
+                                               false);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) |
+                                          static_cast<uint32_t>(lo));
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg_step(double &a, double &b, double &c, int key) {
+    constexpr int kNoKey = -2147483647 - 1; /* below every key (keys are >= -1) */
+    const int k = __builtin_amdgcn_update_dpp(kNoKey, key, CTRL, ROWS, 0xf, false);
+    const double ta = dpp_move<CTRL, ROWS>(a);
+    const double tb = dpp_move<CTRL, ROWS>(b);
+    const double tc = dpp_move<CTRL, ROWS>(c);
+    if (k == key) {
+        a += ta;
+        b += tb;
+        c += tc;
     }
-    return v;
+}
+
+__device__ __forceinline__ void seg_incl_sum3(double &a, double &b, double &c, int key) {
+    seg_step<0x111, 0xf>(a, b, c, key); /* row_shr:1 */
+    seg_step<0x112, 0xf>(a, b, c, key); /* row_shr:2 */
+    seg_step<0x114, 0xf>(a, b, c, key); /* row_shr:4 */
+    seg_step<0x118, 0xf>(a, b, c, key); /* row_shr:8 */
+    seg_step<0x142, 0xa>(a, b, c, key); /* row_bcast:15 into rows 1, 3 */
+    seg_step<0x143, 0xc>(a, b, c, key); /* row_bcast:31 into rows 2, 3 */
 }
 
 /* Per-block table of j-side sums (LDS): slots, and linear probes tried
@@ -1812,7 +1841,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_backward(
     for (int k = 0; k < 9; k++) c[k] = 0.0;
     const int64_t t0 = static_cast<int64_t>(blockIdx.x) * tile;
     const int64_t t1 = t0 + tile < npairs ? t0 + tile : npairs;
-    /* every lane of a wave runs the same trip count so that the shuffles
+    /* every lane of a wave runs the same trip count so that the lane moves
        stay converged; lanes past the tile carry key -1 and zeros */
     for (int64_t wb = t0 + (threadIdx.x & ~(kWave - 1)); wb < t1; wb += kBlock) {
         const int64_t p = wb + lane_id();
@@ -1865,9 +1894,8 @@ __global__ __launch_bounds__(kBlock) void k_pair_backward(
         c[0] += s0 * g0; c[1] += s0 * g1; c[2] += s0 * g2;
         c[3] += s1 * g0; c[4] += s1 * g1; c[5] += s1 * g2;
         c[6] += s2 * g0; c[7] += s2 * g1; c[8] += s2 * g2;
-        const double a0 = seg_incl_sum(g0, i);
-        const double a1 = seg_incl_sum(g1, i);
-        const double a2 = seg_incl_sum(g2, i);
+        double a0 = g0, a1 = g1, a2 = g2;
+        seg_incl_sum3(a0, a1, a2, i);
         const int next = __shfl_down(i, 1, kWave);
         const bool tail = lane_id() == kWave - 1 || next != i;
         if (valid && tail) {
