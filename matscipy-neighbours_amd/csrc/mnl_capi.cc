@@ -142,7 +142,7 @@ struct mnl_plan {
         counts, order_stats, home_list, home_items, item_base,
         offsets, partial, geo, atom_base, cell_base, pt, hkey, total, hpos, hper_atom,
         htypes, cell_partial, det_scratch, tri_m, tri_cnt, tri_off, tri_vidx, tri_vstart, tri_flag, lj,
-        tune_out;
+        tune_out, scan_status;
     int64_t *host_total = nullptr; /* pinned */
     /* state of the last count */
     bool counted = false;
@@ -300,6 +300,9 @@ int ensure_workspaces(mnl_plan *p, hipStream_t stream, int32_t nat, int64_t cell
         {&p->order_stats, 3 * sizeof(int32_t)},
         {&p->home_list, hashed ? n * sizeof(int32_t) : 0},
         {&p->home_items, hashed ? sizeof(int64_t) : 0},
+        /* the single-pass scans' status words (not in the fixed workspace:
+           its scans take the two-kernel form) */
+        {&p->scan_status, p->fixed ? 0 : static_cast<size_t>(scan_status_words()) * 8},
     };
     bool grow = false;
     for (const Need &nd : needs) grow = grow || nd.bytes > nd.b->cap;
@@ -318,6 +321,9 @@ int ensure_workspaces(mnl_plan *p, hipStream_t stream, int32_t nat, int64_t cell
     p->pt_host.clear(); /* buffers may have moved: upload the table again */
     /* no atom order seen yet: the first build keeps the default layout */
     MNL_HIP(launch_fill_i32(p->order_stats.as<int32_t>(), 3, 0, stream));
+    /* the scans keep their status words zero between scans from here on */
+    if (p->scan_status.ptr)
+        MNL_HIP(launch_fill_i64(p->scan_status.as<int64_t>(), scan_status_words(), 0, stream));
     return MNL_SUCCESS;
 }
 
@@ -356,7 +362,7 @@ int enqueue_binning(mnl_plan *p, hipStream_t stream, const double *positions,
     }
     MNL_HIP(launch_scan_i32_to_i32(p->cell_count.as<int32_t>(), cells,
                                    p->cell_first.as<int32_t>(), p->partial.as<int64_t>(),
-                                   nullptr, stream));
+                                   nullptr, stream, p->scan_status.as<uint64_t>()));
     MNL_HIP(launch_scatter(p->raw4.as<int4>(), p->ticket.as<int32_t>(), nat,
                            p->cell_first.as<int32_t>(), p->tmp.as<int32_t>(), stream));
     MNL_HIP(launch_rank_gather(positions, p->raw4.as<int4>(), p->tmp.as<int32_t>(), nat,
@@ -372,7 +378,7 @@ int enqueue_binning(mnl_plan *p, hipStream_t stream, const double *positions,
            the rank/gather), the list and its length */
         MNL_HIP(launch_scan_i32_to_i32(p->ticket.as<int32_t>(), nat, p->tmp.as<int32_t>(),
                                        p->partial.as<int64_t>(), p->home_items.as<int64_t>(),
-                                       stream));
+                                       stream, p->scan_status.as<uint64_t>()));
         MNL_HIP(launch_home_list(p->raw4.as<int4>(), p->ticket.as<int32_t>(),
                                  p->tmp.as<int32_t>(), nat, p->home_list.as<int32_t>(), stream));
     }
@@ -393,11 +399,12 @@ int enqueue_count_scan(mnl_plan *p, hipStream_t stream, int64_t *total_copy) {
         MNL_HIP(launch_scan_sorted_counts(p->scount.as<int32_t>(), p->slot.as<int32_t>(),
                                           p->counts.as<int32_t>(), nat,
                                           p->offsets.as<int64_t>(), p->partial.as<int64_t>(),
-                                          p->total.as<int64_t>(), stream, total_copy));
+                                          p->total.as<int64_t>(), stream, total_copy,
+                                          p->scan_status.as<uint64_t>()));
     else
         MNL_HIP(launch_scan_i32_to_i64(p->counts.as<int32_t>(), nat, p->offsets.as<int64_t>(),
                                        p->partial.as<int64_t>(), p->total.as<int64_t>(),
-                                       stream, total_copy));
+                                       stream, total_copy, p->scan_status.as<uint64_t>()));
     return MNL_SUCCESS;
 }
 
@@ -989,7 +996,8 @@ void mnl_plan_destroy(mnl_plan *p) {
                       &p->partial,   &p->geo,       &p->atom_base, &p->cell_base, &p->pt,
                       &p->hkey,      &p->total,     &p->hpos,   &p->hper_atom, &p->htypes,
                       &p->cell_partial, &p->det_scratch, &p->tri_m,   &p->tri_cnt,  &p->tri_off,
-                      &p->tri_vidx,  &p->tri_vstart, &p->tri_flag, &p->lj, &p->tune_out};
+                      &p->tri_vidx,  &p->tri_vstart, &p->tri_flag, &p->lj, &p->tune_out,
+                      &p->scan_status};
     for (DevBuf *b : bufs) b->release();
     if (p->host_total) (void)hipHostFree(p->host_total);
     delete p;

@@ -155,20 +155,26 @@ hipError_t launch_home_list(const int4 *raw4, const int32_t *flag, const int32_t
 int64_t pair_waves(const PairArgs &a);
 /* Exclusive scan of n int32 values; out gets n (+1 with the total) entries.
    partial needs scan_partial_size(n) int64 entries. total and total2
-   (device, may be NULL) receive the sum. */
+   (device, may be NULL) receive the sum. With status (scan_status_words()
+   words, all zero before the first scan that uses them; each scan leaves
+   them zero), scans of up to 1024 tiles of 4096 values run in one launch
+   with a decoupled look-back instead of a reduce and an apply kernel; the
+   same status words must not serve two scans at once. */
 int64_t scan_partial_size(int64_t n);
+int64_t scan_status_words();
 hipError_t launch_scan_i32_to_i32(const int32_t *in, int64_t n, int32_t *out,
                                   int64_t *partial, int64_t *total,
-                                  hipStream_t stream);
+                                  hipStream_t stream, uint64_t *status = nullptr);
 hipError_t launch_scan_i32_to_i64(const int32_t *in, int64_t n, int64_t *out,
                                   int64_t *partial, int64_t *total,
-                                  hipStream_t stream, int64_t *total2 = nullptr);
+                                  hipStream_t stream, int64_t *total2 = nullptr,
+                                  uint64_t *status = nullptr);
 /* The same scan over per-atom counts held in cell-sorted order (the half
    count): value of atom a = scount[slot[a]], also written to counts[a]. */
 hipError_t launch_scan_sorted_counts(const int32_t *scount, const int32_t *slot,
                                      int32_t *counts, int64_t n, int64_t *out,
                                      int64_t *partial, int64_t *total, hipStream_t stream,
-                                     int64_t *total2 = nullptr);
+                                     int64_t *total2 = nullptr, uint64_t *status = nullptr);
 hipError_t launch_scan_i64_to_i64(const int64_t *in, int64_t n, int64_t *out,
                                   int64_t *partial, int64_t *total,
                                   hipStream_t stream);

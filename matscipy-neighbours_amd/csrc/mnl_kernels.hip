@@ -553,6 +553,153 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const TI *__restrict__ in
     }
 }
 
+/* Decoupled look-back (one wave): publishes tile t's sum, adds up the sums
+   of the tiles before it until one with an inclusive prefix, publishes its
+   own inclusive prefix and returns the exclusive one. Status word: bits
+   62-63 = 0 not yet, 1 sum only, 2 inclusive prefix; bits 0-61 the value.
+   Every tile waited on took its ticket earlier, so it is resident or done
+   and publishes its sum before it waits on anything. */
+__device__ __forceinline__ int64_t tile_lookback(uint64_t *status, int t, int64_t sum,
+                                                 int32_t *flags) {
+    constexpr uint64_t kAgg = uint64_t(1) << 62, kInc = uint64_t(2) << 62;
+    constexpr uint64_t kVal = kAgg - 1;
+    constexpr int kSpinLimit = 1 << 22;
+    const int lane = lane_id();
+    if (t == 0) {
+        if (lane == 0)
+            __hip_atomic_store(&status[0], kInc | static_cast<uint64_t>(sum), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        return 0;
+    }
+    if (lane == 0)
+        __hip_atomic_store(&status[t], kAgg | static_cast<uint64_t>(sum), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    int64_t excl = 0;
+    int j = t - 1;
+    int spins = 0;
+    while (true) {
+        const int idx = j - lane;
+        /* before the first tile: an inclusive zero */
+        const uint64_t v = idx >= 0 ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT)
+                                    : kInc;
+        const uint64_t flag = v >> 62;
+        const uint64_t inc = __builtin_amdgcn_ballot_w64(flag == 2);
+        const uint64_t zero = __builtin_amdgcn_ballot_w64(flag == 0);
+        const int first = inc ? static_cast<int>(__builtin_ctzll(inc)) : kWave;
+        const uint64_t need = first >= kWave - 1 ? ~uint64_t(0) : ((uint64_t(2) << first) - 1);
+        if (zero & need) {
+            if (++spins > kSpinLimit) {
+                /* never expected: report instead of waiting for ever */
+                if (lane == 0) atomicOr(flags, 1);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        excl += wave_sum(lane <= first ? static_cast<int64_t>(v & kVal) : int64_t(0));
+        if (first < kWave) break;
+        j -= kWave;
+    }
+    if (lane == 0)
+        __hip_atomic_store(&status[t], kInc | static_cast<uint64_t>(excl + sum), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    return excl;
+}
+
+/*
+ * Single-pass exclusive scan of nb (2 .. kScanOnepassTiles) tiles: each block
+ * takes a tile by ticket, scans it as k_scan_apply does, and finds the sum
+ * of the tiles before it by a decoupled look-back, so one launch replaces the
+ * reduce and apply kernels. status holds kScanOnepassTiles words and then the
+ * ticket, the count of tiles done and an error flag (int32); all are zero
+ * before the launch, and the last tile to finish its look-back (when no
+ * block reads the words any more) zeroes them again for the next scan, so
+ * the scan records no clearing and replays correctly in a graph.
+ */
+constexpr int kScanOnepassTiles = 1024;
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kBlock) void k_scan_onepass(
+    const TI *__restrict__ in, int64_t n, TO *__restrict__ out, int write_total,
+    int64_t *__restrict__ total, int64_t *__restrict__ total2, const int32_t *__restrict__ gsrc,
+    const int32_t *__restrict__ gslot, int32_t *__restrict__ gout, uint64_t *__restrict__ status,
+    int nb) {
+    __shared__ int64_t s_tile[kScanTile];
+    __shared__ int64_t s_w[kWavesPerBlock];
+    __shared__ int64_t s_pre;
+    __shared__ int64_t s_last;
+    __shared__ int s_t;
+    int32_t *counters = reinterpret_cast<int32_t *>(status + kScanOnepassTiles);
+    if (threadIdx.x == 0) s_t = atomicAdd(&counters[0], 1);
+    __syncthreads();
+    const int t = s_t;
+    const int64_t base = static_cast<int64_t>(t) * kScanTile;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int64_t idx = base + k * kBlock + threadIdx.x;
+        int64_t v = 0;
+        if (idx < n) {
+            if (gslot) {
+                const int32_t c = gsrc[gslot[idx]];
+                gout[idx] = c;
+                v = c;
+            } else {
+                v = static_cast<int64_t>(in[idx]);
+            }
+            if (idx == n - 1) s_last = v;
+        }
+        s_tile[k * kBlock + threadIdx.x] = v;
+    }
+    __syncthreads();
+    int64_t local[kScanItems];
+    int64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        local[k] = sum;
+        sum += s_tile[threadIdx.x * kScanItems + k];
+    }
+    const int64_t incl = wave_incl_scan(sum);
+    if (lane_id() == kWave - 1) s_w[threadIdx.x / kWave] = incl;
+    __syncthreads();
+    if (threadIdx.x < kWave) {
+        int64_t bsum = 0;
+        for (int w = 0; w < kWavesPerBlock; w++) bsum += s_w[w];
+        const int64_t pre = tile_lookback(status, t, bsum, counters + 2);
+        if (threadIdx.x == 0) s_pre = pre;
+        /* the last tile through its look-back clears the status words */
+        int done = 0;
+        if (threadIdx.x == 0) done = atomicAdd(&counters[1], 1);
+        done = __shfl(done, 0, kWave);
+        if (done == nb - 1) {
+            for (int k = threadIdx.x; k < nb; k += kWave) status[k] = 0;
+            if (threadIdx.x == 0) {
+                counters[0] = 0;
+                counters[1] = 0;
+            }
+        }
+    }
+    __syncthreads();
+    int64_t pre = s_pre;
+    for (int w = 0; w < static_cast<int>(threadIdx.x / kWave); w++) pre += s_w[w];
+    pre += incl - sum;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) s_tile[threadIdx.x * kScanItems + k] = pre + local[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int64_t idx = base + k * kBlock + threadIdx.x;
+        if (idx < n) out[idx] = static_cast<TO>(s_tile[k * kBlock + threadIdx.x]);
+    }
+    if (t == nb - 1 && threadIdx.x == kBlock - 1) {
+        /* the last element's exclusive prefix plus itself */
+        const int64_t tot = s_tile[n - 1 - base] + s_last;
+        if (write_total) out[n] = static_cast<TO>(tot);
+        if (total) *total = tot;
+        if (total2) *total2 = tot;
+    }
+}
+
 /* ---------------------------------------------------------------- pairs */
 
 enum { MODE_COUNT = 0, MODE_FILL64 = 1, MODE_FILL32 = 2, MODE_MATRIX = 3 };
@@ -2047,11 +2194,13 @@ hipError_t launch_home_list(const int4 *raw4, const int32_t *flag, const int32_t
 
 int64_t scan_partial_size(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
+int64_t scan_status_words() { return kScanOnepassTiles + 2; }
+
 template <typename TI, typename TO>
 static hipError_t scan_impl(const TI *in, int64_t n, TO *out, int64_t *partial,
                             int64_t *total, int64_t *total2, bool write_total,
                             hipStream_t stream, const int32_t *gsrc = nullptr,
-                            const int32_t *gslot = nullptr) {
+                            const int32_t *gslot = nullptr, uint64_t *status = nullptr) {
     if (n <= 0) {
         if (write_total) {
             /* out[0] = 0 */
@@ -2075,6 +2224,12 @@ static hipError_t scan_impl(const TI *in, int64_t n, TO *out, int64_t *partial,
                                                        gslot ? gout : nullptr);
         return hipGetLastError();
     }
+    if (status && nb <= kScanOnepassTiles) {
+        k_scan_onepass<TI, TO><<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(
+            in, n, out, write_total ? 1 : 0, total, total2, gslot ? gsrc : nullptr, gslot,
+            gslot ? gout : nullptr, status, static_cast<int>(nb));
+        return hipGetLastError();
+    }
     if (gslot)
         k_scan_reduce_gather<<<static_cast<unsigned>(nb), kBlock, 0, stream>>>(gsrc, gslot, gout,
                                                                                n, partial);
@@ -2092,22 +2247,25 @@ static hipError_t scan_impl(const TI *in, int64_t n, TO *out, int64_t *partial,
 }
 
 hipError_t launch_scan_i32_to_i32(const int32_t *in, int64_t n, int32_t *out,
-                                  int64_t *partial, int64_t *total, hipStream_t stream) {
-    return scan_impl<int32_t, int32_t>(in, n, out, partial, total, nullptr, false, stream);
+                                  int64_t *partial, int64_t *total, hipStream_t stream,
+                                  uint64_t *status) {
+    return scan_impl<int32_t, int32_t>(in, n, out, partial, total, nullptr, false, stream,
+                                       nullptr, nullptr, status);
 }
 
 hipError_t launch_scan_i32_to_i64(const int32_t *in, int64_t n, int64_t *out,
                                   int64_t *partial, int64_t *total, hipStream_t stream,
-                                  int64_t *total2) {
-    return scan_impl<int32_t, int64_t>(in, n, out, partial, total, total2, true, stream);
+                                  int64_t *total2, uint64_t *status) {
+    return scan_impl<int32_t, int64_t>(in, n, out, partial, total, total2, true, stream,
+                                       nullptr, nullptr, status);
 }
 
 hipError_t launch_scan_sorted_counts(const int32_t *scount, const int32_t *slot,
                                      int32_t *counts, int64_t n, int64_t *out,
                                      int64_t *partial, int64_t *total, hipStream_t stream,
-                                     int64_t *total2) {
+                                     int64_t *total2, uint64_t *status) {
     return scan_impl<int32_t, int64_t>(counts, n, out, partial, total, total2, true, stream,
-                                       scount, slot);
+                                       scount, slot, status);
 }
 
 hipError_t launch_scan_i64_to_i64(const int64_t *in, int64_t n, int64_t *out,
