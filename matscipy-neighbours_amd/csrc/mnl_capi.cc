@@ -529,8 +529,23 @@ int enqueue_count(mnl_plan *p, hipStream_t stream, const double origin[3],
 int read_total(mnl_plan *p, hipStream_t stream, int64_t *npairs) {
     MNL_HIP(hipMemcpyAsync(p->host_total, p->total.ptr, sizeof(int64_t),
                            hipMemcpyDeviceToHost, stream));
+    /* the single-pass scans' error flag (set only by a look-back that gave up
+       waiting, never expected), read with the total */
+    int32_t *flag = p->scan_status.ptr ? p->scan_status.as<int32_t>() +
+                                             2 * scan_status_words() - 2
+                                       : nullptr;
+    if (flag)
+        MNL_HIP(hipMemcpyAsync(p->host_total + 1, flag, sizeof(int32_t), hipMemcpyDeviceToHost,
+                               stream));
     MNL_HIP(hipStreamSynchronize(stream));
     p->npairs = *p->host_total;
+    if (flag && *reinterpret_cast<int32_t *>(p->host_total + 1) != 0) {
+        p->counted = false;
+        /* status words and flag back to zero, so that the next build can run */
+        MNL_HIP(launch_fill_i64(p->scan_status.as<int64_t>(), scan_status_words(), 0, stream));
+        return fail("internal error: a scan's look-back gave up waiting; the build's "
+                    "offsets are wrong");
+    }
     p->counted = true;
     if (npairs) *npairs = p->npairs;
     return MNL_SUCCESS;
@@ -977,7 +992,7 @@ int mnl_plan_create(int device_id, mnl_plan **plan) {
     auto *p = new mnl_plan();
     p->device = dev;
     void *h = nullptr;
-    hipError_t e = hipHostMalloc(&h, sizeof(int64_t), hipHostMallocDefault);
+    hipError_t e = hipHostMalloc(&h, 2 * sizeof(int64_t), hipHostMallocDefault);
     if (e != hipSuccess) {
         delete p;
         return fail(std::string("hipHostMalloc failed: ") + hipGetErrorString(e));

@@ -590,9 +590,11 @@ __device__ __forceinline__ int64_t tile_lookback(uint64_t *status, int t, int64_
         const uint64_t need = first >= kWave - 1 ? ~uint64_t(0) : ((uint64_t(2) << first) - 1);
         if (zero & need) {
             if (++spins > kSpinLimit) {
-                /* never expected: report instead of waiting for ever */
+                /* never expected: report instead of waiting for ever. The
+                   flag stays set (the host checks it where it reads a pair
+                   total), and the prefix 0 keeps every later write in bounds. */
                 if (lane == 0) atomicOr(flags, 1);
-                break;
+                return 0;
             }
             __builtin_amdgcn_s_sleep(1);
             continue;
