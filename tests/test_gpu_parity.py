@@ -193,6 +193,39 @@ def test_lattice_at_shell_distance(gpu, rc):
     _check(pos, 6 * np.eye(3), True, float(rc), backends=(0, 1))
 
 
+@pytest.mark.parametrize("pbc", [True, (True, False, True)])
+def test_pairs_within_ulps_of_the_cutoff(gpu, pbc):
+    """Simple-cubic lattice at spacing a = rc = 1.7 with every coordinate
+    nudged by up to 3 ulps: hundreds of nearest-neighbour pairs have r2
+    within a few ulps of rc^2, on both sides. The count's fast test cannot
+    decide those and re-tests them in the reference's operation order; the
+    counts and rows must still be the oracle's."""
+    rng = np.random.default_rng(77)
+    a = 1.7
+    g = np.arange(7, dtype=float) * a
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pos = pos + rng.integers(-3, 4, pos.shape) * np.spacing(pos + a)
+    cell = 7 * a * np.eye(3)
+    # nearest-neighbour pairs along the axes inside the box: within 1e-13 of rc^2
+    d = pos[:, None, :] - pos[None, :, :]
+    r2 = np.einsum("ijk,ijk->ij", d, d)
+    near = np.abs(r2 - a * a) < 1e-13
+    assert near.sum() > 1000
+    assert (r2[near] < a * a).any() and (r2[near] >= a * a).any()
+    ref = _check(pos, cell, pbc, a, backends=(0, 1), msg=str(pbc))
+    assert 0 < len(ref["i"])
+
+
+def test_multi_image_pairs_at_the_cutoff(gpu):
+    """Two atoms in a 1.7 A cubic box with rc exactly two and three box
+    lengths: images at r2 == rc^2 and within ulps of it, reached through
+    cell offsets of several box lengths."""
+    cell = 1.7 * np.eye(3)
+    pos = np.array([[0.0, 0.0, 0.0], [0.85, 0.85, 0.85 + 2 * np.spacing(0.85)]])
+    for rc in (3.4, 5.1):
+        _check(pos, cell, True, rc, backends=(0, 1), msg=f"rc={rc}")
+
+
 def test_atoms_on_cell_faces(gpu):
     """Atoms exactly on bin faces k*bin (floor of the cell index decides)."""
     cell = 8.0 * np.eye(3)
