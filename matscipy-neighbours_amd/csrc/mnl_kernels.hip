@@ -359,8 +359,13 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
     const double *__restrict__ per_atom, const int32_t *__restrict__ types,
     double4 *__restrict__ spos, int4 *__restrict__ sinfo, int32_t *__restrict__ stype,
     int32_t *__restrict__ scount, int32_t *__restrict__ slot, int32_t *__restrict__ first_flag) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nat) return;
+    /* one thread per slot of the ticket-ordered member lists: the record of
+       the member held there goes to a slot of the same cell, so a wave
+       writes one contiguous range of records for any input order (a thread
+       per atom scatters them when the input order is spatially random) */
+    const int s0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s0 >= nat) return;
+    const int a = tmp[s0];
     const int4 rc = raw4[a];
     const int b = cell_first[rc.w];
     const int m = cell_count[rc.w];
