@@ -1075,6 +1075,38 @@ __device__ __forceinline__ bool pair_test(const PairArgs &A, const HomeCell &H, 
 }
 
 /*
+ * Per-type cutoffs with the table in LDS, by home-atom class. The home atoms
+ * of a group are taken class by class: their type, or one class for types
+ * outside the table. For a class each candidate lane holds the limit
+ * min(rc^2, table[type][tj]) in a register, and the test is then one
+ * comparison, as for a uniform cutoff. This is pair_test's per-type rule:
+ * (r2 < rc^2) & (r2 < t) is r2 < min(rc^2, t), and a pair with either type
+ * outside the table keeps r2 < rc^2 alone (neighbour_visit.hh:100-108). The
+ * order in which home atoms are taken does not change the output: each
+ * atom's pairs keep their rank in its own row.
+ */
+__device__ __forceinline__ int home_class(const PairArgs &A, int ti) {
+    return (ti >= 0 && ti < A.ncut) ? ti : A.ncut;
+}
+
+__device__ __forceinline__ double class_limit(const PairArgs &A, const HomeCell &H,
+                                              lds_dptr pt, int v, int tj) {
+    const bool in = v < A.ncut && tj >= 0 && tj < A.ncut;
+    const double t = pt[in ? v * A.ncut + tj : 0];
+    return in ? fmin(H.cutoff_sq, t) : H.cutoff_sq;
+}
+
+/* pair_test with the lane's limit for the home atom's class */
+__device__ __forceinline__ bool pair_test_lim(const Cand &C, const double4 &pi, double lim,
+                                              double &d0, double &d1, double &d2) {
+    d0 = (C.px - pi.x) + C.o0;
+    d1 = (C.py - pi.y) + C.o1;
+    d2 = (C.pz - pi.z) + C.o2;
+    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    return r2 < lim;
+}
+
+/*
  * Count pass. One wave per home cell. The home cell's neighbour cells, in
  * the reference's traversal order (z outer, y, x inner, each -n..n;
  * neighbour_visit.hh:63-84), are concatenated into one candidate index
@@ -1095,6 +1127,9 @@ __device__ __forceinline__ void count_cell(const PairArgs &A, int h, int part, S
     home_part(H.m, part, H.split, lo, hi);
     for (int grp0 = lo; grp0 < hi; grp0 += kWave) {
         const int mg = uniform(min(kWave, hi - grp0));
+        const int hcls = CUT == CUT_PER_TYPE_LDS && lane < mg
+                             ? home_class(A, A.stype[H.hb + grp0 + lane])
+                             : 0;
         int acc = 0;
         for (int sg = 0; sg < H.nseg; sg += kWave) {
             const int nsg = min(kWave, H.nseg - sg);
@@ -1106,15 +1141,33 @@ __device__ __forceinline__ void count_cell(const PairArgs &A, int h, int part, S
                 load_candidate<CUT, false>(A, H, S, sg, nsg, cb + lane, total, C);
                 const int c = cb + lane;
                 uint32_t mlo = 0, mhi = 0;
-                for (int t = 0; t < mg; t++) {
-                    const int si = H.hb + grp0 + t;
-                    const double4 pi = load_const(A.spos, si);
-                    double d0, d1, d2;
-                    const bool ok = pair_test<CUT>(A, H, C, si, pi, d0, d1, d2, pt_sq) &
-                                    (c != self_base + t);
+                auto home = [&](int t, bool ok) {
+                    ok = ok & (c != self_base + t);
                     const uint64_t mask = __builtin_amdgcn_ballot_w64(ok);
                     mlo = write_lane(mlo, static_cast<uint32_t>(mask), t);
                     mhi = write_lane(mhi, static_cast<uint32_t>(mask >> 32), t);
+                };
+                if constexpr (CUT == CUT_PER_TYPE_LDS) {
+                    /* home atoms class by class, each lane's limit per class */
+                    for (int v = 0; v <= A.ncut; v++) {
+                        uint64_t todo = __builtin_amdgcn_ballot_w64(lane < mg && hcls == v);
+                        if (!todo) continue;
+                        const double lim = class_limit(A, H, pt_sq, v, C.tj);
+                        while (todo) {
+                            const int t = static_cast<int>(__builtin_ctzll(todo));
+                            todo &= todo - 1;
+                            const double4 pi = load_const(A.spos, H.hb + grp0 + t);
+                            double d0, d1, d2;
+                            home(t, pair_test_lim(C, pi, lim, d0, d1, d2));
+                        }
+                    }
+                } else {
+                    for (int t = 0; t < mg; t++) {
+                        const int si = H.hb + grp0 + t;
+                        const double4 pi = load_const(A.spos, si);
+                        double d0, d1, d2;
+                        home(t, pair_test<CUT>(A, H, C, si, pi, d0, d1, d2, pt_sq));
+                    }
                 }
                 acc += __popc(mlo) + __popc(mhi);
             }
@@ -1153,14 +1206,13 @@ __device__ __forceinline__ int add_lane_bit(int acc, uint64_t mask) {
    mask over the chunk's low lanes. */
 template <int CUT, bool CENTRE>
 __device__ __forceinline__ void half_sweep(const PairArgs &A, const HomeCell &H, const Cand &C,
-                                           int grp0, int mg, int cb, lds_dptr pt_sq,
+                                           int grp0, int mg, int cb, lds_dptr pt_sq, int hcls,
                                            uint32_t &icnt, int &cj) {
-    for (int t = 0; t < mg; t++) {
+    auto home = [&](int t, auto &&test) {
         const int si = H.hb + grp0 + t;
         const double4 pi = load_const(A.spos, si);
         double d0, d1, d2;
-        uint64_t mask =
-            __builtin_amdgcn_ballot_w64(pair_test<CUT>(A, H, C, si, pi, d0, d1, d2, pt_sq));
+        uint64_t mask = __builtin_amdgcn_ballot_w64(test(si, pi, d0, d1, d2));
         if (CENTRE) {
             const int hi = min(grp0 + t, H.m - 1) - cb; /* last lane to drop */
             if (hi >= kWave - 1)
@@ -1170,6 +1222,25 @@ __device__ __forceinline__ void half_sweep(const PairArgs &A, const HomeCell &H,
         }
         cj = add_lane_bit(cj, mask);
         icnt = write_lane(icnt, static_cast<uint32_t>(__popcll(mask)), t);
+    };
+    if constexpr (CUT == CUT_PER_TYPE_LDS) {
+        for (int v = 0; v <= A.ncut; v++) {
+            uint64_t todo = __builtin_amdgcn_ballot_w64(lane_id() < mg && hcls == v);
+            if (!todo) continue;
+            const double lim = class_limit(A, H, pt_sq, v, C.tj);
+            while (todo) {
+                const int t = static_cast<int>(__builtin_ctzll(todo));
+                todo &= todo - 1;
+                home(t, [&](int, const double4 &pi, double &d0, double &d1, double &d2) {
+                    return pair_test_lim(C, pi, lim, d0, d1, d2);
+                });
+            }
+        }
+    } else {
+        for (int t = 0; t < mg; t++)
+            home(t, [&](int si, const double4 &pi, double &d0, double &d1, double &d2) {
+                return pair_test<CUT>(A, H, C, si, pi, d0, d1, d2, pt_sq);
+            });
     }
 }
 
@@ -1183,6 +1254,9 @@ __device__ __forceinline__ void count_cell_half(const PairArgs &A, int h, int pa
     home_part(H.m, part, H.split, lo, hi);
     for (int grp0 = lo; grp0 < hi; grp0 += kWave) {
         const int mg = uniform(min(kWave, hi - grp0));
+        const int hcls = CUT == CUT_PER_TYPE_LDS && lane < mg
+                             ? home_class(A, A.stype[H.hb + grp0 + lane])
+                             : 0;
         int acc = 0;
         for (int sg = H.centre; sg < H.nseg; sg += kWave) {
             const int nsg = min(kWave, H.nseg - sg);
@@ -1196,9 +1270,9 @@ __device__ __forceinline__ void count_cell_half(const PairArgs &A, int h, int pa
                 /* the centre segment leads the first group: its candidates
                    c < m are the home atoms, taken only after the home atom */
                 if (sg == H.centre && cb < H.m)
-                    half_sweep<CUT, true>(A, H, C, grp0, mg, cb, pt_sq, icnt, cj);
+                    half_sweep<CUT, true>(A, H, C, grp0, mg, cb, pt_sq, hcls, icnt, cj);
                 else
-                    half_sweep<CUT, false>(A, H, C, grp0, mg, cb, pt_sq, icnt, cj);
+                    half_sweep<CUT, false>(A, H, C, grp0, mg, cb, pt_sq, hcls, icnt, cj);
                 acc += static_cast<int>(icnt);
                 if (cj) atomicAdd(A.scount + C.j, cj);
             }
@@ -1526,6 +1600,9 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, int part, Fi
             L.hbz[lane] = inf.z;
             L.hi[lane] = inf.w;
         }
+        const int hcls = CUT == CUT_PER_TYPE_LDS && lane < mg
+                             ? home_class(A, A.stype[H.hb + grp0 + lane])
+                             : 0;
         wave_sync_lds();
         int acc = 0;     /* lane t: pairs of atom t emitted so far */
         int head = 0;    /* ring positions (uniform) */
@@ -1542,46 +1619,101 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, int part, Fi
                     if (w0 + k * kWave < total)
                         load_candidate<CUT, true>(A, H, L.seg, sg, nsg, w0 + k * kWave + lane,
                                                   total, C[k]);
-                for (int t = 0; t < mg; t++) {
-                    const int si = H.hb + grp0 + t;
-                    const double4 pi = load_const(A.spos, si);
-                    /* atom t's pairs so far: a scalar across the window */
-                    int rank0 = __builtin_amdgcn_readlane(acc, t);
+                if constexpr (CUT == CUT_PER_TYPE_LDS) {
+                    /* home atoms class by class, each lane's limit per class */
+                    for (int v = 0; v <= A.ncut; v++) {
+                        uint64_t todo = __builtin_amdgcn_ballot_w64(lane < mg && hcls == v);
+                        if (!todo) continue;
+                        double lim[FC];
 #pragma unroll
-                    for (int k = 0; k < FC; k++) {
-                        const int cb = w0 + k * kWave;
-                        if (cb >= total) break;
-                        double d0, d1, d2;
-                        uint64_t mask = __builtin_amdgcn_ballot_w64(
-                            pair_test<CUT>(A, H, C[k], si, pi, d0, d1, d2, pt_sq));
-                        /* the atom itself at zero offset (neighbour_visit.hh:86) */
-                        const int self_lane = self_base + t - cb;
-                        if (self_lane >= 0 && self_lane < kWave) mask &= ~(1ull << self_lane);
-                        if (mask) {
-                            if (__builtin_amdgcn_inverse_ballot_w64(mask)) {
-                                const int r = mbcnt64(mask);
-                                const int e = (head + r) & (kRing - 1);
-                                L.rdx[e] = d0;
-                                L.rdy[e] = d1;
-                                L.rdz[e] = d2;
-                                L.rj[e] = C[k].j;
-                                L.rc0[e] = C[k].c0;
-                                L.rc1[e] = C[k].c1;
-                                L.rc2[e] = C[k].c2;
-                                L.rt[e] = t;
-                                L.rrank[e] = rank0 + r;
+                        for (int k = 0; k < FC; k++)
+                            lim[k] = class_limit(A, H, pt_sq, v, C[k].tj);
+                        while (todo) {
+                            const int t = static_cast<int>(__builtin_ctzll(todo));
+                            todo &= todo - 1;
+                            const int si = H.hb + grp0 + t;
+                            const double4 pi = load_const(A.spos, si);
+                            /* atom t's pairs so far: a scalar across the window */
+                            int rank0 = __builtin_amdgcn_readlane(acc, t);
+#pragma unroll
+                            for (int k = 0; k < FC; k++) {
+                                const int cb = w0 + k * kWave;
+                                if (cb >= total) break;
+                                double d0, d1, d2;
+                                uint64_t mask = __builtin_amdgcn_ballot_w64(
+                                    pair_test_lim(C[k], pi, lim[k], d0, d1, d2));
+                                /* the atom itself at zero offset (neighbour_visit.hh:86) */
+                                const int self_lane = self_base + t - cb;
+                                if (self_lane >= 0 && self_lane < kWave) mask &= ~(1ull << self_lane);
+                                if (mask) {
+                                    if (__builtin_amdgcn_inverse_ballot_w64(mask)) {
+                                        const int r = mbcnt64(mask);
+                                        const int e = (head + r) & (kRing - 1);
+                                        L.rdx[e] = d0;
+                                        L.rdy[e] = d1;
+                                        L.rdz[e] = d2;
+                                        L.rj[e] = C[k].j;
+                                        L.rc0[e] = C[k].c0;
+                                        L.rc1[e] = C[k].c1;
+                                        L.rc2[e] = C[k].c2;
+                                        L.rt[e] = t;
+                                        L.rrank[e] = rank0 + r;
+                                    }
+                                    const int pc = static_cast<int>(__popcll(mask));
+                                    rank0 += pc;
+                                    head += pc;
+                                    if (head - tail >= kWave) {
+                                        flush_ring<MODE>(A, H, L, tail, kWave);
+                                        tail += kWave;
+                                    }
+                                }
                             }
-                            const int pc = static_cast<int>(__popcll(mask));
-                            rank0 += pc;
-                            head += pc;
-                            if (head - tail >= kWave) {
-                                flush_ring<MODE>(A, H, L, tail, kWave);
-                                tail += kWave;
-                            }
+                            acc = static_cast<int>(
+                                write_lane(static_cast<uint32_t>(acc), static_cast<uint32_t>(rank0), t));
                         }
                     }
-                    acc = static_cast<int>(
-                        write_lane(static_cast<uint32_t>(acc), static_cast<uint32_t>(rank0), t));
+                } else {
+                    for (int t = 0; t < mg; t++) {
+                        const int si = H.hb + grp0 + t;
+                        const double4 pi = load_const(A.spos, si);
+                        /* atom t's pairs so far: a scalar across the window */
+                        int rank0 = __builtin_amdgcn_readlane(acc, t);
+#pragma unroll
+                        for (int k = 0; k < FC; k++) {
+                            const int cb = w0 + k * kWave;
+                            if (cb >= total) break;
+                            double d0, d1, d2;
+                            uint64_t mask = __builtin_amdgcn_ballot_w64(
+                                pair_test<CUT>(A, H, C[k], si, pi, d0, d1, d2, pt_sq));
+                            /* the atom itself at zero offset (neighbour_visit.hh:86) */
+                            const int self_lane = self_base + t - cb;
+                            if (self_lane >= 0 && self_lane < kWave) mask &= ~(1ull << self_lane);
+                            if (mask) {
+                                if (__builtin_amdgcn_inverse_ballot_w64(mask)) {
+                                    const int r = mbcnt64(mask);
+                                    const int e = (head + r) & (kRing - 1);
+                                    L.rdx[e] = d0;
+                                    L.rdy[e] = d1;
+                                    L.rdz[e] = d2;
+                                    L.rj[e] = C[k].j;
+                                    L.rc0[e] = C[k].c0;
+                                    L.rc1[e] = C[k].c1;
+                                    L.rc2[e] = C[k].c2;
+                                    L.rt[e] = t;
+                                    L.rrank[e] = rank0 + r;
+                                }
+                                const int pc = static_cast<int>(__popcll(mask));
+                                rank0 += pc;
+                                head += pc;
+                                if (head - tail >= kWave) {
+                                    flush_ring<MODE>(A, H, L, tail, kWave);
+                                    tail += kWave;
+                                }
+                            }
+                        }
+                        acc = static_cast<int>(
+                            write_lane(static_cast<uint32_t>(acc), static_cast<uint32_t>(rank0), t));
+                    }
                 }
             }
             __builtin_amdgcn_wave_barrier();
