@@ -1743,8 +1743,23 @@ __device__ __forceinline__ void fill_cell(const PairArgs &A, int h, int part, Fi
     }
 }
 
+/* Launch bounds of the fill: at least 4 waves per SIMD (at most 128 VGPRs)
+   where the compiler would otherwise take a few registers more and run 3,
+   and the cap costs no spills: the 5-chunk uniform-cutoff fills other than
+   the dense float64 one (float32 outputs on fcc 1M: 1016 -> 936 us per
+   step) and the hashed 4-chunk fills (per-type slab on the hashed grid:
+   326 -> 313 us). Elsewhere the compiler's own allocation stays: capping
+   the dense float64 and dense 4-chunk fills changes their code for a
+   slight loss, and the 5-chunk per-atom and per-type fills would spill. */
 template <int MODE, int CUT, bool HASHED, int FC>
-__global__ __launch_bounds__(kBlock) void k_fill(PairArgs A) {
+constexpr int fill_min_waves() {
+    if (FC == 4) return HASHED ? 4 : 1;
+    return (CUT == CUT_GLOBAL && (MODE != MODE_FILL64 || HASHED)) ? 4 : 1;
+}
+
+template <int MODE, int CUT, bool HASHED, int FC>
+__global__ __launch_bounds__(kBlock, (fill_min_waves<MODE, CUT, HASHED, FC>())) void k_fill(
+    PairArgs A) {
     __shared__ FillLds s_lds[kWavesPerBlock];
     __shared__ double s_pt[CUT == CUT_PER_TYPE_LDS ? kLdsTypes : 1];
     const lds_dptr pt_sq = stage_types<CUT>(A, s_pt);
