@@ -1793,13 +1793,15 @@ int mnl_debug_sqrt(const double *x_host, double *y_host, int64_t n) {
     if (n <= 0) return MNL_SUCCESS;
     if (check_device_available() != MNL_SUCCESS) return MNL_ERROR;
     void *dx = nullptr, *dy = nullptr;
-    MNL_HIP(hipMalloc(&dx, size_t(n) * 8));
-    MNL_HIP(hipMalloc(&dy, size_t(n) * 8));
-    MNL_HIP(hipMemcpy(dx, x_host, size_t(n) * 8, hipMemcpyHostToDevice));
-    MNL_HIP(launch_sqrt_probe(static_cast<double *>(dx), static_cast<double *>(dy), n, nullptr));
-    MNL_HIP(hipMemcpy(y_host, dy, size_t(n) * 8, hipMemcpyDeviceToHost));
-    (void)hipFree(dx);
-    (void)hipFree(dy);
+    hipError_t e = hipMalloc(&dx, size_t(n) * 8);
+    if (e == hipSuccess) e = hipMalloc(&dy, size_t(n) * 8);
+    if (e == hipSuccess) e = hipMemcpy(dx, x_host, size_t(n) * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_sqrt_probe(static_cast<double *>(dx), static_cast<double *>(dy), n, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(y_host, dy, size_t(n) * 8, hipMemcpyDeviceToHost);
+    if (dx) (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    if (e != hipSuccess) return fail(std::string("mnl_debug_sqrt: ") + hipGetErrorString(e));
     return MNL_SUCCESS;
 }
 
