@@ -2102,6 +2102,94 @@ constexpr int kHashSlots = 1024;
 constexpr int kHashProbe = 8;
 
 /*
+ * Adds the block's table of j-side sums to the global ones, in ascending j.
+ * The table holds its atoms in hash order; added in that order, the 64
+ * atomics of a wave land on about as many cache lines of each of gx, gy,
+ * gz. Sorted, consecutive lanes take consecutive atoms of the index runs a
+ * block's neighbourhoods consist of (measured on bcc 131k: 333 -> 212 us for
+ * the backward with the flush's atomics on consecutive addresses).
+ *   1. Every thread keeps its kHashSlots / kBlock slots (key, sums) in
+ *      registers.
+ *   2. The slot numbers are sorted by key (bitonic, in the first sum array's
+ *      LDS: key in the high word, empty slots last, slot number low).
+ *   3. s_key[slot] = sorted position; each slot's owner writes its sums
+ *      there, and thread t adds sorted positions t, t + kBlock, ...
+ * Called by all threads of the block after the table's last update.
+ */
+__device__ __forceinline__ void flush_table_sorted(int *s_key, double (*s_g)[kHashSlots],
+                                                   double *__restrict__ gx,
+                                                   double *__restrict__ gy,
+                                                   double *__restrict__ gz) {
+    constexpr int kPer = kHashSlots / kBlock;
+    static_assert(kPer * kBlock == kHashSlots, "whole slots per thread");
+    int key[kPer];
+    double v0[kPer], v1[kPer], v2[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const int k = static_cast<int>(threadIdx.x) + r * kBlock;
+        key[r] = s_key[k];
+        v0[r] = s_g[0][k];
+        v1[r] = s_g[1][k];
+        v2[r] = s_g[2][k];
+    }
+    __syncthreads();
+    uint64_t *s_sort = reinterpret_cast<uint64_t *>(s_g[0]);
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const int k = static_cast<int>(threadIdx.x) + r * kBlock;
+        /* an empty slot (-1) gets the largest key: it sorts last */
+        s_sort[k] = (static_cast<uint64_t>(static_cast<uint32_t>(key[r])) << 32) |
+                    static_cast<uint32_t>(k);
+    }
+    for (int size = 2; size <= kHashSlots; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < kHashSlots / 2; e += kBlock) {
+                const int lo = 2 * e - (e & (stride - 1));
+                const int hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const uint64_t a = s_sort[lo], b = s_sort[hi];
+                if ((a > b) == up) {
+                    s_sort[lo] = b;
+                    s_sort[hi] = a;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    int skey[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const int pos = static_cast<int>(threadIdx.x) + r * kBlock;
+        const uint64_t e = s_sort[pos];
+        skey[r] = static_cast<int>(static_cast<uint32_t>(e >> 32));
+        s_key[static_cast<int>(e & 0xffffffffu)] = pos;
+    }
+    __syncthreads();
+    int mine[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; r++) mine[r] = s_key[static_cast<int>(threadIdx.x) + r * kBlock];
+    /* every read of s_sort (the first sum array) ended before the barrier */
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        s_g[0][mine[r]] = v0[r];
+        s_g[1][mine[r]] = v1[r];
+        s_g[2][mine[r]] = v2[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const int pos = static_cast<int>(threadIdx.x) + r * kBlock;
+        const int j = skey[r];
+        if (j >= 0) {
+            unsafeAtomicAdd(&gx[j], s_g[0][pos]);
+            unsafeAtomicAdd(&gy[j], s_g[1][pos]);
+            unsafeAtomicAdd(&gz[j], s_g[2][pos]);
+        }
+    }
+}
+
+/*
  * Backward of D_p = r[j] - r[i] + S_p . cell and d_p = |D_p| over an
  * i-sorted pair list. Per pair g = dL/dD + (dL/dd) D / d (the d term is
  * dropped for coincident atoms, d = 0). Each block takes a contiguous tile
@@ -2206,14 +2294,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_backward(
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < kHashSlots; k += kBlock) {
-        const int jk = s_key[k];
-        if (jk >= 0) {
-            unsafeAtomicAdd(&gx[jk], s_g[0][k]);
-            unsafeAtomicAdd(&gy[jk], s_g[1][k]);
-            unsafeAtomicAdd(&gz[jk], s_g[2][k]);
-        }
-    }
+    flush_table_sorted(s_key, s_g, gx, gy, gz);
 #pragma unroll
     for (int k = 0; k < 9; k++) c[k] = wave_sum(c[k]);
     const int w = threadIdx.x / kWave;
