@@ -77,6 +77,30 @@ def test_backward_matches_torch_autograd(gpu, bcc):
     assert _rel_err(cell.grad, gc_ref) < 1e-10
 
 
+def test_backward_shuffled_atoms(gpu, bcc):
+    """The bcc atoms in random order: the rows of a block's tile are atoms
+    far apart, so its LDS table of j-side sums fills (every slot used, then
+    pairs adding straight to global memory), and the sorted flush adds a full
+    table. Gradients against torch autograd as above."""
+    import torch
+
+    from matscipy_neighbours_amd.autograd import neighbour_list_autograd
+    perm = np.random.default_rng(11).permutation(len(bcc["positions"]))
+    shuffled = {"positions": bcc["positions"][perm], "cell": bcc["cell"]}
+    pos, cell = _setup(shuffled)
+    i, j, D, d, S = neighbour_list_autograd("ijDdS", pos, cell, True, RC)
+    P = len(i)
+    print(f"shuffled bcc: {pos.shape[0]} atoms, {P} pairs", flush=True)
+    assert 8.0e6 < P < 9.0e6
+    w = torch.randn((P, 3), dtype=torch.float64, device="cuda:0", generator=torch.Generator(
+        device="cuda:0").manual_seed(4))
+    L = _loss(d) + (w * D).sum()
+    L.backward()
+    _, _, gp_ref, gc_ref = _autograd_reference(pos, cell, i, j, S, w)
+    assert _rel_err(pos.grad, gp_ref) < 1e-10
+    assert _rel_err(cell.grad, gc_ref) < 1e-10
+
+
 def test_backward_d_only_and_D_only(gpu, bcc):
     """Each incoming gradient alone (the other is None)."""
     import torch
