@@ -98,6 +98,23 @@ def test_per_type_out_of_range_types_keep_global_test(gpu):
                                                [1.2, 0.9, 1.7]]), numbers=types)
 
 
+@pytest.mark.parametrize("nt", [11, 12])
+def test_per_type_many_types(gpu, nt):
+    """11 types (121 table entries, the largest table kept in LDS, where home
+    atoms are taken class by class) and 12 (the table read from global
+    memory), with types -1 and nt outside the table, symmetric tables with
+    entries down to 0.3 rc; cell by cell the home atoms of one cell carry
+    many classes."""
+    rng = np.random.default_rng(100 + nt)
+    n = 3000
+    pos = rng.uniform(0, 14.0, (n, 3))
+    types = rng.integers(-1, nt + 1, n).astype(np.int32)
+    t = rng.uniform(0.6, 2.0, (nt, nt))
+    table = (t + t.T) / 2
+    _check(pos, 14.0 * np.eye(3), True, table, numbers=types, backends=(0, 1),
+           msg=f"{nt} types")
+
+
 @pytest.mark.parametrize("L", [8.0, 2.5])
 def test_per_type_asymmetric_table(gpu, L):
     """A per-type matrix with cut[a][b] != cut[b][a]: the pair (i, j) uses
