@@ -289,7 +289,16 @@ class _Build:
         first, then i, j, S, each part aligned to its element size), so that a
         host result needs a single transfer. ``need_total=False`` (no
         quantities) leaves the pair total unread, so the call does not wait
-        for the device."""
+        for the device.
+
+        Device outputs of a build like the plan's previous one (same atom
+        count and cutoff, as in an MD loop): the fill is queued right behind
+        the count into buffers sized from the previous pair total (plus 1%
+        and 4096), and the total is read once, after the fill; the outputs
+        are the first P rows of those buffers. A total above that capacity
+        fills again into buffers of the exact size. This saves the wait
+        between count and fill. Host results (``packed``) keep the exact
+        sizes, since their copy to the host dominates."""
         torch = self.torch
         lib = _lib.load()
         idx = self.dev.index if self.dev.index is not None else torch.cuda.current_device()
@@ -297,47 +306,78 @@ class _Build:
         out = {}
         with plan.lock, torch.cuda.device(idx):
             stream = torch.cuda.current_stream(idx).cuda_stream
-            npairs = _lib.ctypes.c_int64(0)
-            args = (plan.handle, _lib.ctypes.c_void_p(stream), _lib.ptr(self.co),
+            sp = _lib.ctypes.c_void_p(stream)
+            args = (plan.handle, sp, _lib.ptr(self.co),
                     _lib.ptr(self.ce), _lib.ptr(self.inv), _lib.ptr(self.pbc), self.nat,
                     _lib.ptr(self.pos), self.rc, _lib.ptr(self.per_atom_t),
                     _lib.ptr(self.per_type_sq), self.nc, _lib.ptr(self.types_t),
                     int(self.backend))
-            if flags or need_total:
+            fdt = torch.float32 if f32 else torch.float64
+            kw = dict(device=self.dev)
+            key = (self.nat, repr(self.rc))
+            last = getattr(plan, "last_build", None)
+            if flags and not packed and last is not None and last[0] == key:
+                np_dev = torch.empty(1, dtype=torch.int64, **kw)
+                _lib.check(lib.mnl_plan_count_async(*args, _lib.ptr(np_dev)))
+                cap = last[1] + last[1] // 100 + 4096
+                bufs = self._alloc(flags, cap, fdt, kw, packed=False)
+                self._fill_async(lib, plan, sp, cap, bufs, f32)
+                P = int(np_dev.item())
+                if P > cap:
+                    bufs = self._alloc(flags, P, fdt, kw, packed=False)
+                    self._fill_async(lib, plan, sp, P, bufs, f32)
+                for q, t in bufs.items():
+                    out[q] = t[:P]
+            elif flags or need_total:
+                npairs = _lib.ctypes.c_int64(0)
                 _lib.check(lib.mnl_plan_count(*args, _lib.ctypes.byref(npairs)))
                 P = int(npairs.value)
+                out = self._alloc(flags, P, fdt, kw, packed)
+                if flags:
+                    fill = lib.mnl_plan_fill_f32 if f32 else lib.mnl_plan_fill
+                    _lib.check(fill(plan.handle, sp, _lib.ptr(out.get("i")),
+                                    _lib.ptr(out.get("j")), _lib.ptr(out.get("D")),
+                                    _lib.ptr(out.get("d")), _lib.ptr(out.get("S"))))
             else:
                 _lib.check(lib.mnl_plan_count_async(*args, None))
                 P = None
-            fdt = torch.float32 if f32 else torch.float64
-            kw = dict(device=self.dev)
-            parts = [(q, shape, dt) for q, bit, shape, dt in (
-                ("D", 4, (P, 3), fdt), ("d", 8, (P,), fdt), ("i", 1, (P,), torch.int32),
-                ("j", 2, (P,), torch.int32), ("S", 16, (P, 3), torch.int32)) if flags & bit]
-            if packed:
-                sizes = [int(np.prod(shape)) * (torch.empty((), dtype=dt).element_size())
-                         for _, shape, dt in parts]
-                buf = torch.empty(sum(sizes), dtype=torch.uint8, **kw)
-                off = 0
-                for (q, shape, dt), nb in zip(parts, sizes):
-                    out[q] = buf[off:off + nb].view(dt).view(shape)
-                    off += nb
-                out["_packed"] = buf
-            else:
-                for q, shape, dt in parts:
-                    out[q] = torch.empty(shape, dtype=dt, **kw)
-            if flags:
-                fill = lib.mnl_plan_fill_f32 if f32 else lib.mnl_plan_fill
-                _lib.check(fill(plan.handle, _lib.ctypes.c_void_p(stream),
-                                _lib.ptr(out.get("i")), _lib.ptr(out.get("j")),
-                                _lib.ptr(out.get("D")), _lib.ptr(out.get("d")),
-                                _lib.ptr(out.get("S"))))
+            if P is not None:
+                plan.last_build = (key, P)
             if want_counts:
                 out["c"] = torch.empty(self.nat, dtype=torch.int32, **kw)
-                _lib.check(lib.mnl_plan_counts(plan.handle, _lib.ctypes.c_void_p(stream),
-                                               _lib.ptr(out["c"])))
+                _lib.check(lib.mnl_plan_counts(plan.handle, sp, _lib.ptr(out["c"])))
         self.npairs = P
         return out
+
+    @staticmethod
+    def _alloc(flags, n, fdt, kw, packed):
+        """Output tensors of n rows for the quantities in ``flags``: separate
+        tensors, or views of one packed buffer (see ``run``)."""
+        torch = _torch()
+        parts = [(q, shape, dt) for q, bit, shape, dt in (
+            ("D", 4, (n, 3), fdt), ("d", 8, (n,), fdt), ("i", 1, (n,), torch.int32),
+            ("j", 2, (n,), torch.int32), ("S", 16, (n, 3), torch.int32)) if flags & bit]
+        out = {}
+        if packed:
+            sizes = [int(np.prod(shape)) * (torch.empty((), dtype=dt).element_size())
+                     for _, shape, dt in parts]
+            buf = torch.empty(sum(sizes), dtype=torch.uint8, **kw)
+            off = 0
+            for (q, shape, dt), nb in zip(parts, sizes):
+                out[q] = buf[off:off + nb].view(dt).view(shape)
+                off += nb
+            out["_packed"] = buf
+        else:
+            for q, shape, dt in parts:
+                out[q] = torch.empty(shape, dtype=dt, **kw)
+        return out
+
+    @staticmethod
+    def _fill_async(lib, plan, sp, capacity, bufs, f32):
+        _lib.check(lib.mnl_plan_fill_async(plan.handle, sp, int(capacity),
+                                           _lib.ptr(bufs.get("i")), _lib.ptr(bufs.get("j")),
+                                           _lib.ptr(bufs.get("D")), _lib.ptr(bufs.get("d")),
+                                           _lib.ptr(bufs.get("S")), 1 if f32 else 0))
 
 
 def _is_f32(out_dtype):
