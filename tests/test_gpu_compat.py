@@ -170,3 +170,31 @@ def test_coordination_matches_oracle_back_to_back(gpu):
         assert np.array_equal(g.cpu().numpy(), want)
         assert np.array_equal(coordination(positions=p, cell=cell, pbc=(True, True, False),
                                            cutoff=2.7), want)
+
+
+def test_repeated_device_builds_with_growing_totals(gpu):
+    """Device builds that repeat the plan's previous one (same atom count and
+    cutoff) queue the fill with a capacity from the previous pair total: a
+    second build with about the same total fits, a third in a box half the
+    size (eight times the pairs) overflows it and fills again at the exact
+    size. Every result equals the oracle bit for bit, and the returned rows
+    are exactly the pair total."""
+    import torch
+
+    import oracle
+    from conftest import assert_same_pairs
+    from matscipy_neighbours_amd import _lib, neighbour_list
+    if hasattr(_lib.plan_for(0), "last_build"):
+        del _lib.plan_for(0).last_build
+    rng = np.random.default_rng(45)
+    frac = rng.uniform(0.0, 1.0, (4000, 3))
+    for k, edge in enumerate((24.0, 24.0, 12.0)):
+        cell = edge * np.eye(3)
+        pos = frac @ cell + (rng.normal(0.0, 0.01, frac.shape) if k == 1 else 0.0)
+        got = neighbour_list("ijdDS", positions=torch.from_numpy(pos).to("cuda:0"), cell=cell,
+                             pbc=True, cutoff=3.0)
+        got = dict(zip("ijdDS", (t.cpu().numpy() for t in got)))
+        inv = np.ascontiguousarray(np.linalg.inv(cell.T))
+        ref = oracle.neighbour_list("ijdDS", np.zeros(3), cell, inv, np.ones(3, bool), pos, 3.0)
+        assert_same_pairs(got, ref, msg=f"build {k}")
+        assert len(got["i"]) == len(ref["i"])
