@@ -99,7 +99,7 @@ def test_slab_per_type_bitwise(gpu, backend):
 
 
 def test_many_atoms_row_offsets(gpu):
-    """4.6 M atoms (more than the 4096 x 1024 counts one scan pass adds up
+    """4.6 M atoms (more than the 2048 x 1024 counts one scan pass adds up
     in its apply blocks, so the three-kernel scan runs), uniform at 0.01
     atoms/A^3, rc 3 A: the rows follow the per-atom counts (i sorted, each
     atom's row length equal to its count, the total equal to the sum), and

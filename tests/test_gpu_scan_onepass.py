@@ -1,5 +1,5 @@
 """The single-pass scans (one launch with a decoupled look-back for scans of
-2 to 1024 tiles of 4096 values) inside plan builds, against the oracle.
+2 to 1024 tiles of 2048 values) inside plan builds, against the oracle.
 
 Their status words are cleared by the last tile of each scan, so the
 sequence of builds on one plan matters: sizes of one, two, three and more
@@ -55,7 +55,7 @@ def test_scan_sizes_in_sequence(gpu):
     from matscipy_neighbours_amd import _lib
     lib = _lib.load()
     plan = _lib.Plan(0)
-    for k, n in enumerate((3000, 5000, 4096 * 2 + 7, 4096 * 4 + 100, 8192, 4096)):
+    for k, n in enumerate((1500, 2500, 2048 * 2 + 7, 2048 * 4 + 100, 4096, 2048)):
         pos, cell = _uniform(n, 0.05, 100 + k)
         print(f"build {k}: {n} atoms", flush=True)
         res = _plan_build(lib, _lib, plan, torch.from_numpy(pos).cuda(), cell, 4.0)
@@ -67,14 +67,14 @@ def test_scan_sizes_in_sequence(gpu):
 
 @pytest.mark.parametrize("tiles", [1024, 1025])
 def test_largest_single_pass_scan(gpu, tiles):
-    """4096 x 1024 atoms (the largest single-pass scan) and one tile more
+    """2048 x 1024 atoms (the largest single-pass scan) and one tile more
     (the reduce/spine/apply form), sparse enough for few pairs: the rows
     follow the per-atom counts exactly."""
     import torch
 
     from matscipy_neighbours_amd import _lib
     lib = _lib.load()
-    n = 4096 * tiles - 5
+    n = 2048 * tiles - 5
     pos, cell = _uniform(n, 0.01, tiles)
     print(f"{n} atoms", flush=True)
     plan = _lib.Plan(0)
@@ -88,7 +88,7 @@ def test_largest_single_pass_scan(gpu, tiles):
 
 
 def test_graph_replays_multi_tile(gpu):
-    """count_async + fill_async of 20,000 atoms (five tiles in each scan)
+    """count_async + fill_async of 20,000 atoms (ten tiles in the pair-count scan)
     captured in one graph and replayed three times with the atoms moved:
     every replay equals an eager build (the status words the scans clear
     after themselves are zero again for each replay)."""
