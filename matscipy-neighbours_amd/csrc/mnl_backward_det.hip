@@ -225,7 +225,13 @@ __device__ __forceinline__ double lane_tree(double v) {
 }
 
 /* One wave per atom: its j-side entries (ascending pair order) minus its
-   own row, each as 64 strided lane sums combined by the fixed tree. */
+   own row, each as 64 strided lane sums combined by the fixed tree. Lane l
+   adds entries l, l + 64, l + 128, ... of each row in that order, starting
+   from 0.0. Each pass of the loop takes two such entries of both rows and
+   issues the row loads and the permutation loads together before any sum,
+   so a row of up to 128 entries costs three dependent memory round trips
+   (starts, permutation with own row, gathered entries) rather than one
+   per 64 entries and per load. */
 __global__ __launch_bounds__(kBlock) void k_det_positions(int32_t nat,
                                                           const int64_t *__restrict__ jstart,
                                                           const int32_t *__restrict__ jperm,
@@ -235,18 +241,59 @@ __global__ __launch_bounds__(kBlock) void k_det_positions(int32_t nat,
     const int64_t a = static_cast<int64_t>(blockIdx.x) * kWaves + threadIdx.x / kWave;
     if (a >= nat) return;
     const int lane = threadIdx.x & (kWave - 1);
+    const int64_t jend = jstart[a + 1], iend = istart[a + 1];
+    int64_t k = jstart[a] + lane, q = istart[a] + lane;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int64_t k = jstart[a] + lane; k < jstart[a + 1]; k += kWave) {
-        const int64_t p = jperm[k];
-        s0 += G[3 * p + 0];
-        s1 += G[3 * p + 1];
-        s2 += G[3 * p + 2];
-    }
     double r0 = 0.0, r1 = 0.0, r2 = 0.0;
-    for (int64_t q = istart[a] + lane; q < istart[a + 1]; q += kWave) {
-        r0 += G[3 * q + 0];
-        r1 += G[3 * q + 1];
-        r2 += G[3 * q + 2];
+    while (k < jend || q < iend) {
+        const bool jA = k < jend, jB = k + kWave < jend;
+        const bool iA = q < iend, iB = q + kWave < iend;
+        const int64_t pA = jA ? static_cast<int64_t>(jperm[k]) : 0;
+        const int64_t pB = jB ? static_cast<int64_t>(jperm[k + kWave]) : 0;
+        double qa0 = 0.0, qa1 = 0.0, qa2 = 0.0, qb0 = 0.0, qb1 = 0.0, qb2 = 0.0;
+        if (iA) {
+            qa0 = G[3 * q + 0];
+            qa1 = G[3 * q + 1];
+            qa2 = G[3 * q + 2];
+        }
+        if (iB) {
+            qb0 = G[3 * (q + kWave) + 0];
+            qb1 = G[3 * (q + kWave) + 1];
+            qb2 = G[3 * (q + kWave) + 2];
+        }
+        double ga0 = 0.0, ga1 = 0.0, ga2 = 0.0, gb0 = 0.0, gb1 = 0.0, gb2 = 0.0;
+        if (jA) {
+            ga0 = G[3 * pA + 0];
+            ga1 = G[3 * pA + 1];
+            ga2 = G[3 * pA + 2];
+        }
+        if (jB) {
+            gb0 = G[3 * pB + 0];
+            gb1 = G[3 * pB + 1];
+            gb2 = G[3 * pB + 2];
+        }
+        if (jA) {
+            s0 += ga0;
+            s1 += ga1;
+            s2 += ga2;
+        }
+        if (jB) {
+            s0 += gb0;
+            s1 += gb1;
+            s2 += gb2;
+        }
+        if (iA) {
+            r0 += qa0;
+            r1 += qa1;
+            r2 += qa2;
+        }
+        if (iB) {
+            r0 += qb0;
+            r1 += qb1;
+            r2 += qb2;
+        }
+        k += 2 * kWave;
+        q += 2 * kWave;
     }
     s0 = lane_tree(s0);
     s1 = lane_tree(s1);
